@@ -1,0 +1,936 @@
+// api.hip -- C ABI of libmfc_lbm: context life cycle, host/device transfers,
+// the state machine that lets the reference's separate subroutine calls run
+// as the fused pull-scheme kernels, and the host half of FORCING.
+//
+// State machine.  The resident buffer holds post-collision values; the
+// reference's `f` is either that buffer itself (phase PRE_STREAM, between
+// collision and streaming) or its pulled view (phase POST_STREAM, after
+// streaming).  `streaming` only flips the phase.  The reference's rho/ux/
+// uy/uz arrays are either materialised (MOM_ARRAYS: uploaded, initpop,
+// rhoupdat) or virtual (MOM_VIRTUAL: set by macrovar, recomputed inside the
+// collide kernel from the pulled field and the forcing captured at
+// macrovar), valid while the field is unchanged since that macrovar.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "internal.h"
+
+using namespace mfc;
+
+struct mfc_ctx {
+  Ctx c;
+};
+
+namespace {
+
+constexpr int kModeRing = 8;
+constexpr int kTimingPool = 8192;
+
+int fail(mfc_ctx* ctx, int code, const char* what, hipError_t e = hipSuccess) {
+  if (ctx) {
+    char buf[512];
+    if (e != hipSuccess)
+      snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+    else
+      snprintf(buf, sizeof buf, "%s", what);
+    ctx->c.err = buf;
+  }
+  return code;
+}
+
+#define HIPCHK(expr)                                        \
+  do {                                                      \
+    hipError_t e_ = (expr);                                 \
+    if (e_ != hipSuccess) return fail(ctx, MFC_ERR_HIP, #expr, e_); \
+  } while (0)
+
+#define NEEDCTX \
+  if (!ctx) return MFC_ERR_ARG
+
+__global__ void k_set_status(int32_t* st, int idx, int val) { st[idx] = val; }
+__global__ void k_set_double(double* d, double v) { d[0] = v; }
+__global__ void k_fill_i32(int32_t* a, uint64_t n, int32_t v) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+__global__ void k_fill_f64(double* a, uint64_t n, double v) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+hipError_t set_status(Ctx& c, int idx, int val) {
+  hipLaunchKernelGGL(k_set_status, dim3(1), dim3(1), 0, c.stream, c.d_status, idx, val);
+  return hipGetLastError();
+}
+
+// ---- FORCING, host half (gaussian.f90:182-306) ----------------------------
+// Park-Miller minimal standard generator with Bays-Durham shuffle
+// (Numerical Recipes ran1), as ranff/ranpp (gaussian.f90:374-404).
+double ran1(int32_t* idum, int32_t* iy, int32_t* iv) {
+  const int32_t IA = 16807, IM = 2147483647, IQ = 127773, IR = 2836, NTAB = 32;
+  const int32_t NDIV = 1 + (IM - 1) / NTAB;
+  const double AM = 1.0 / IM, RNMX = 1.0 - 1.2e-7;
+  if (*idum <= 0 || *iy == 0) {
+    *idum = std::max(-*idum, (int32_t)1);
+    for (int j = NTAB + 8; j >= 1; --j) {
+      const int32_t k = *idum / IQ;
+      *idum = IA * (*idum - k * IQ) - IR * k;
+      if (*idum < 0) *idum += IM;
+      if (j <= NTAB) iv[j - 1] = *idum;
+    }
+    *iy = iv[0];
+  }
+  const int32_t k = *idum / IQ;
+  *idum = IA * (*idum - k * IQ) - IR * k;
+  if (*idum < 0) *idum += IM;
+  const int32_t j = 1 + *iy / NDIV;
+  *iy = iv[j - 1];
+  iv[j - 1] = *idum;
+  const double r = AM * (double)*iy;
+  return r < RNMX ? r : RNMX;
+}
+
+inline int mix(int i, int j, int k) { return (i - 1) + 6 * ((j - 1) + 5 * (k - 1)); }
+inline double kx_of(int i) { return (double)(i / 2); }
+inline double kyz_of(int j) { return j <= 3 ? (double)(j - 1) : (double)(j - 6); }
+
+// kx = 0 plane made Hermitian so the physical field is real
+void hermitian_kx0(double* q) {
+  for (int k = 1; k <= 5; ++k) {
+    const int kk = (k == 1) ? 1 : 7 - k;
+    for (int j = 2; j <= 3; ++j) {
+      const int jj = 7 - j;
+      q[mix(1, jj, kk)] = q[mix(1, j, k)];
+      q[mix(2, jj, kk)] = -q[mix(2, j, k)];
+    }
+  }
+  for (int k = 2; k <= 3; ++k) {
+    const int kk = 7 - k;
+    q[mix(1, 1, kk)] = q[mix(1, 1, k)];
+    q[mix(2, 1, kk)] = -q[mix(2, 1, k)];
+  }
+}
+
+// One Euler step of the Ornstein-Uhlenbeck mode process with white-noise
+// increments, solenoidal projection and Hermitian symmetry.
+void forcing_modes_host(Ctx& c) {
+  double fr[3][150];
+  const double var = 447.31 * (c.p.vscale * c.p.tscale) * (c.p.vscale * c.p.tscale);
+  const double tf = 0.038 / c.p.tscale, dt = 1.0, smallx = 1.0e-18;
+  for (int mc = 0; mc < 3; ++mc)
+    for (int k = 1; k <= 5; ++k)
+      for (int j = 1; j <= 5; ++j)
+        for (int i = 1; i <= 6; i += 2) {
+          double c1 = ran1(&c.iseedf, &c.iyf, c.ivf);
+          if (c1 < 1.0e-6) c1 = 1.0e-6;
+          const double rmag = sqrt(-(4.0 * var * dt * log(c1) / tf));
+          const double rph = ran1(&c.iseedf, &c.iyf, c.ivf);
+          fr[mc][mix(i, j, k)] = rmag * cos(c.p.pi2 * rph);
+          fr[mc][mix(i + 1, j, k)] = rmag * sin(c.p.pi2 * rph);
+        }
+  for (int mc = 0; mc < 3; ++mc) {
+    hermitian_kx0(fr[mc]);
+    fr[mc][mix(1, 1, 1)] = 0.0;
+    fr[mc][mix(2, 1, 1)] = 0.0;
+  }
+  for (int k = 1; k <= 5; ++k)
+    for (int j = 1; j <= 5; ++j)
+      for (int i = 1; i <= 6; ++i) {
+        const int a = mix(i, j, k);
+        const double kx = kx_of(i), ky = kyz_of(j), kz = kyz_of(k);
+        const double radm = sqrt(kx * kx + ky * ky + kz * kz);
+        if (radm < c.p.xkf) {
+          for (int mc = 0; mc < 3; ++mc) c.bmod[mc][a] = c.bmod[mc][a] * (1.0 - dt / tf) + fr[mc][a];
+          const double t1 = kx * c.bmod[0][a], t2 = ky * c.bmod[1][a], t3 = kz * c.bmod[2][a];
+          double divb = (t1 + t2 + t3) / ((radm + smallx) * (radm + smallx));
+          if (radm == 0.0) divb = 0.0;
+          c.amod[0][a] = c.bmod[0][a] - kx * divb;
+          c.amod[1][a] = c.bmod[1][a] - ky * divb;
+          c.amod[2][a] = c.bmod[2][a] - kz * divb;
+        } else {
+          c.amod[0][a] = c.amod[1][a] = c.amod[2][a] = 0.0;
+        }
+      }
+  for (int mc = 0; mc < 3; ++mc) {
+    c.amod[mc][mix(1, 1, 1)] = 0.0;
+    c.amod[mc][mix(2, 1, 1)] = 0.0;
+    hermitian_kx0(c.amod[mc]);
+  }
+}
+
+// Upload c.amod into a fresh forcing table slot and make it current.
+int publish_modes(mfc_ctx* ctx) {
+  Ctx& c = ctx->c;
+  int slot = 0;
+  while (slot == c.ft.cur || slot == c.ft.mac) ++slot;
+  bool live = false;
+  for (int mc = 0; mc < 3; ++mc)
+    for (int a = 0; a < 150; ++a) live = live || c.amod[mc][a] != 0.0;
+  c.ft.live[slot] = live ? 1 : 0;
+  if (live) {
+    const int r = c.mode_slot;
+    c.mode_slot = (c.mode_slot + 1) % kModeRing;
+    HIPCHK(hipEventSynchronize(c.mode_ev[r]));
+    double* h = c.h_modes + 450 * r;
+    memcpy(h, c.amod, sizeof(double) * 450);
+    HIPCHK(hipMemcpyAsync(c.ft.d_modes, h, sizeof(double) * 450, hipMemcpyHostToDevice, c.stream));
+    HIPCHK(hipEventRecord(c.mode_ev[r], c.stream));
+    HIPCHK(launch_force_table(c, slot));
+  }
+  c.ft.cur = slot;
+  return MFC_OK;
+}
+
+int ensure_macro(mfc_ctx* ctx) {
+  Ctx& c = ctx->c;
+  if (c.rho) return MFC_OK;
+  const size_t b = sizeof(double) * c.g.ncell;
+  HIPCHK(hipMalloc(&c.rho, b));
+  HIPCHK(hipMalloc(&c.ux, b));
+  HIPCHK(hipMalloc(&c.uy, b));
+  HIPCHK(hipMalloc(&c.uz, b));
+  const unsigned nb = (unsigned)((c.g.ncell + 255) / 256);
+  double* arrs[4] = {c.rho, c.ux, c.uy, c.uz};
+  for (double* a : arrs) hipLaunchKernelGGL(k_fill_f64, dim3(nb), dim3(256), 0, c.stream, a, (uint64_t)c.g.ncell, 0.0);
+  HIPCHK(hipGetLastError());
+  return MFC_OK;
+}
+
+// Make the rho/u arrays hold the reference's values (materialising a
+// virtual macrovar if needed).
+int need_arrays(mfc_ctx* ctx) {
+  Ctx& c = ctx->c;
+  if (c.mom == MOM_ARRAYS) return MFC_OK;
+  if (c.mom == MOM_VIRTUAL) {
+    if (c.mom_version != c.f_version)
+      return fail(ctx, MFC_ERR_STATE, "rho/u requested after the field changed since macrovar");
+    int rc = ensure_macro(ctx);
+    if (rc) return rc;
+    HIPCHK(launch_macro(c, c.phase == PHASE_POST_STREAM));
+    c.mom = MOM_ARRAYS;
+    return MFC_OK;
+  }
+  int rc = ensure_macro(ctx);
+  if (rc) return rc;
+  c.mom = MOM_ARRAYS;
+  return MFC_OK;
+}
+
+int ensure_staging(mfc_ctx* ctx, size_t bytes) {
+  Ctx& c = ctx->c;
+  if (c.staging_bytes >= bytes) return MFC_OK;
+  if (c.staging) HIPCHK(hipFree(c.staging));
+  c.staging = nullptr;
+  c.staging_bytes = 0;
+  HIPCHK(hipMalloc(&c.staging, bytes));
+  c.staging_bytes = bytes;
+  return MFC_OK;
+}
+
+int slab_planes(const Ctx& c) {
+  const size_t plane = (size_t)c.g.nx * c.g.ny * 19 * sizeof(double);
+  const size_t cap = (size_t)256 << 20;
+  int n = (int)std::max<size_t>(1, cap / plane);
+  return std::min(n, c.g.nz);
+}
+
+bool particles_on(const Ctx& c) { return c.pt.np > 0; }
+
+int check_faults(mfc_ctx* ctx) {
+  Ctx& c = ctx->c;
+  HIPCHK(hipMemcpyAsync(c.h_status, c.d_status, sizeof(int32_t) * ST_COUNT, hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  const int32_t* s = c.h_status;
+  char buf[256];
+  if (s[ST_OVERLAP] > 0) {
+    snprintf(buf, sizeof buf, "fault: %d particles overlap at istep = %d", s[ST_OVERLAP], s[ST_OVERLAP_STEP]);
+    return fail(ctx, MFC_ERR_OVERLAP, buf);
+  }
+  if (s[ST_LINK_OVERFLOW] > 0) return fail(ctx, MFC_ERR_CAPACITY, "fault: boundary-link pool exceeded");
+  if (s[ST_NONFINITE] > 0 || s[ST_BOX_FAULT] > 0)
+    return fail(ctx, MFC_ERR_NONFINITE, "fault: particle position not finite or outside the box");
+  return MFC_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+int mfc_create(const mfc_params* p, mfc_ctx** out) {
+  if (!p || !out) return MFC_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MFC_ERR_NODEVICE;
+  if (p->nx < 4 || p->ny < 4 || p->nz < 4) return MFC_ERR_ARG;
+  const uint64_t ncell = (uint64_t)p->nx * p->ny * p->nz;
+  if (ncell >= (1ull << 31)) return MFC_ERR_ARG;
+  if (p->precision != MFC_PREC_FP64 && p->precision != MFC_PREC_FP32) return MFC_ERR_ARG;
+  mfc_ctx* ctx = new (std::nothrow) mfc_ctx();
+  if (!ctx) return MFC_ERR_ARG;
+  Ctx& c = ctx->c;
+  c.p = *p;
+  c.g.nx = p->nx;
+  c.g.ny = p->ny;
+  c.g.nz = p->nz;
+  c.g.walls_y = p->walls_y ? 1 : 0;
+  c.g.ncell = ncell;
+  c.cc = CollideConsts{p->s1, p->s2, p->s4, p->s9, p->s10, p->s13, p->s16, p->omegepsl, p->omegepslj,
+                       p->omegxx, p->ww0, p->ww1, p->ww2, p->body_force[0], p->body_force[1],
+                       p->body_force[2]};
+  c.prec = p->precision;
+  c.tsize = c.prec == MFC_PREC_FP64 ? 8 : 4;
+  c.iseedf = p->iseedf;
+  int rc = MFC_OK;
+  auto bail = [&](int code) {
+    mfc_destroy(ctx);
+    return code;
+  };
+#define CK(expr)                                                         \
+  do {                                                                   \
+    hipError_t e_ = (expr);                                              \
+    if (e_ != hipSuccess) { rc = fail(ctx, MFC_ERR_HIP, #expr, e_); fprintf(stderr, "mfc_create: %s\n", ctx->c.err.c_str()); return bail(rc); } \
+  } while (0)
+  CK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  c.own_stream = true;
+  const size_t pb = (size_t)19 * ncell * c.tsize;
+  CK(hipMalloc(&c.pop[0], pb));
+  CK(hipMalloc(&c.pop[1], pb));
+  CK(hipMemsetAsync(c.pop[0], 0, pb, c.stream));
+  CK(hipMemsetAsync(c.pop[1], 0, pb, c.stream));
+  const size_t gb = sizeof(double) * 18 * (size_t)p->ny * p->nz;
+  for (int s = 0; s < 3; ++s) CK(hipMalloc(&c.ft.g[s], gb));
+  c.ft.cur = 0;
+  c.ft.mac = 0;
+  CK(hipMalloc(&c.ft.d_modes, sizeof(double) * 450));
+  {
+    std::vector<double> ph(2 * (p->nx + p->ny + p->nz));
+    double* px = ph.data();
+    double* py = px + 2 * p->nx;
+    double* pz = py + 2 * p->ny;
+    for (int m = 0; m < p->nx; ++m) { px[2 * m] = cos(p->pi2 * m / p->nx); px[2 * m + 1] = sin(p->pi2 * m / p->nx); }
+    for (int m = 0; m < p->ny; ++m) { py[2 * m] = cos(p->pi2 * m / p->ny); py[2 * m + 1] = sin(p->pi2 * m / p->ny); }
+    for (int m = 0; m < p->nz; ++m) { pz[2 * m] = cos(p->pi2 * m / p->nz); pz[2 * m + 1] = sin(p->pi2 * m / p->nz); }
+    CK(hipMalloc(&c.ft.phx, sizeof(double) * ph.size()));
+    c.ft.phy = c.ft.phx + 2 * p->nx;
+    c.ft.phz = c.ft.phy + 2 * p->ny;
+    CK(hipMemcpy(c.ft.phx, ph.data(), sizeof(double) * ph.size(), hipMemcpyHostToDevice));
+  }
+  CK(hipHostMalloc((void**)&c.h_modes, sizeof(double) * 450 * kModeRing, hipHostMallocDefault));
+  for (int r = 0; r < kModeRing; ++r) CK(hipEventCreateWithFlags(&c.mode_ev[r], hipEventDisableTiming));
+  CK(hipMalloc(&c.d_status, sizeof(int32_t) * ST_COUNT));
+  CK(hipMemsetAsync(c.d_status, 0, sizeof(int32_t) * ST_COUNT, c.stream));
+  CK(hipHostMalloc((void**)&c.h_status, sizeof(int32_t) * ST_COUNT, hipHostMallocDefault));
+  CK(hipMalloc(&c.d_red, sizeof(double) * 8));
+  CK(hipHostMalloc((void**)&c.h_red, sizeof(double) * 8, hipHostMallocDefault));
+  CK(hipStreamSynchronize(c.stream));
+#undef CK
+  *out = ctx;
+  return MFC_OK;
+}
+
+int mfc_destroy(mfc_ctx* ctx) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  if (c.stream) hipStreamSynchronize(c.stream);
+  auto F = [](void* p) {
+    if (p) hipFree(p);
+  };
+  F(c.pop[0]); F(c.pop[1]);
+  F(c.rho); F(c.ux); F(c.uy); F(c.uz);
+  for (int s = 0; s < 3; ++s) F(c.ft.g[s]);
+  F(c.ft.d_modes); F(c.ft.phx);
+  Particles& P = c.pt;
+  double* pd[] = {P.yp, P.ypglb, P.ypglbp, P.wp, P.wpp, P.omgp, P.omgpp, P.fHIp, P.flubp, P.torqp,
+                  P.forcep, P.forcepp, P.torqpp, P.dwdt, P.domgdt, P.thetap, P.map_pos[0], P.map_pos[1],
+                  P.lk_alpha, P.lk_out, P.scratch};
+  for (double* a : pd) F(a);
+  F(P.isn[0]); F(P.isn[1]); F(P.mask[0]); F(P.mask[1]);
+  F(P.lk_node); F(P.lk_dir); F(P.lk_cnt); F(P.lk_off); F(P.sol_cnt);
+  F(c.d_status); F(c.d_red); F(c.staging);
+  if (c.h_modes) hipHostFree(c.h_modes);
+  if (c.h_status) hipHostFree(c.h_status);
+  if (c.h_red) hipHostFree(c.h_red);
+  for (int r = 0; r < kModeRing; ++r)
+    if (c.mode_ev[r]) hipEventDestroy(c.mode_ev[r]);
+  for (auto e : c.tpool) hipEventDestroy(e);
+  c.tpool.clear();
+  if (c.stream && c.own_stream) hipStreamDestroy(c.stream);
+  delete ctx;
+  return MFC_OK;
+}
+
+const char* mfc_last_error(const mfc_ctx* ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+
+int mfc_sync(mfc_ctx* ctx) {
+  NEEDCTX;
+  return check_faults(ctx);
+}
+
+int mfc_status(mfc_ctx* ctx, int32_t* out16) {
+  NEEDCTX;
+  if (!out16) return MFC_ERR_ARG;
+  Ctx& c = ctx->c;
+  HIPCHK(hipMemcpyAsync(c.h_status, c.d_status, sizeof(int32_t) * ST_COUNT, hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  memcpy(out16, c.h_status, sizeof(int32_t) * ST_COUNT);
+  return MFC_OK;
+}
+
+int mfc_set_stream(mfc_ctx* ctx, void* s) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  HIPCHK(hipStreamSynchronize(c.stream));
+  if (c.own_stream) HIPCHK(hipStreamDestroy(c.stream));
+  c.stream = (hipStream_t)s;
+  c.own_stream = false;
+  return MFC_OK;
+}
+
+void* mfc_get_stream(mfc_ctx* ctx) { return ctx ? (void*)ctx->c.stream : nullptr; }
+
+// ---- transfers -------------------------------------------------------------
+int mfc_upload_f(mfc_ctx* ctx, const double* f) {
+  NEEDCTX;
+  if (!f) return MFC_ERR_ARG;
+  Ctx& c = ctx->c;
+  const int ns = slab_planes(c);
+  const size_t plane = (size_t)c.g.nx * c.g.ny * 19;
+  int rc = ensure_staging(ctx, plane * ns * sizeof(double));
+  if (rc) return rc;
+  for (int z0 = 0; z0 < c.g.nz; z0 += ns) {
+    const int nz = std::min(ns, c.g.nz - z0);
+    HIPCHK(hipMemcpyAsync(c.staging, f + plane * z0, plane * nz * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    HIPCHK(launch_aos_to_pop(c, (const double*)c.staging, z0, nz, c.phase == PHASE_POST_STREAM));
+    HIPCHK(hipStreamSynchronize(c.stream));
+  }
+  c.f_version++;
+  return MFC_OK;
+}
+
+int mfc_download_f(mfc_ctx* ctx, double* f) {
+  NEEDCTX;
+  if (!f) return MFC_ERR_ARG;
+  Ctx& c = ctx->c;
+  const int ns = slab_planes(c);
+  const size_t plane = (size_t)c.g.nx * c.g.ny * 19;
+  int rc = ensure_staging(ctx, plane * ns * sizeof(double));
+  if (rc) return rc;
+  for (int z0 = 0; z0 < c.g.nz; z0 += ns) {
+    const int nz = std::min(ns, c.g.nz - z0);
+    HIPCHK(launch_pop_to_aos(c, (double*)c.staging, z0, nz, c.phase == PHASE_POST_STREAM));
+    HIPCHK(hipMemcpyAsync(f + plane * z0, c.staging, plane * nz * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+  }
+  return MFC_OK;
+}
+
+int mfc_upload_macro(mfc_ctx* ctx, const double* rho, const double* ux, const double* uy, const double* uz) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  if (c.mom == MOM_VIRTUAL) {
+    int rc = need_arrays(ctx);
+    if (rc) return rc;
+  }
+  int rc = ensure_macro(ctx);
+  if (rc) return rc;
+  const size_t b = sizeof(double) * c.g.ncell;
+  if (rho) HIPCHK(hipMemcpyAsync(c.rho, rho, b, hipMemcpyHostToDevice, c.stream));
+  if (ux) HIPCHK(hipMemcpyAsync(c.ux, ux, b, hipMemcpyHostToDevice, c.stream));
+  if (uy) HIPCHK(hipMemcpyAsync(c.uy, uy, b, hipMemcpyHostToDevice, c.stream));
+  if (uz) HIPCHK(hipMemcpyAsync(c.uz, uz, b, hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  c.mom = MOM_ARRAYS;
+  return MFC_OK;
+}
+
+int mfc_download_macro(mfc_ctx* ctx, double* rho, double* ux, double* uy, double* uz) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  int rc = need_arrays(ctx);
+  if (rc) return rc;
+  const size_t b = sizeof(double) * c.g.ncell;
+  if (rho) HIPCHK(hipMemcpyAsync(rho, c.rho, b, hipMemcpyDeviceToHost, c.stream));
+  if (ux) HIPCHK(hipMemcpyAsync(ux, c.ux, b, hipMemcpyDeviceToHost, c.stream));
+  if (uy) HIPCHK(hipMemcpyAsync(uy, c.uy, b, hipMemcpyDeviceToHost, c.stream));
+  if (uz) HIPCHK(hipMemcpyAsync(uz, c.uz, b, hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return MFC_OK;
+}
+
+int mfc_download_force(mfc_ctx* ctx, double* fx, double* fy, double* fz) {
+  NEEDCTX;
+  if (!fx || !fy || !fz) return MFC_ERR_ARG;
+  Ctx& c = ctx->c;
+  const size_t b = sizeof(double) * c.g.ncell;
+  int rc = ensure_staging(ctx, 3 * b);
+  if (rc) return rc;
+  double* s = (double*)c.staging;
+  HIPCHK(launch_force_field(c, c.ft.cur, s, s + c.g.ncell, s + 2 * c.g.ncell));
+  HIPCHK(hipMemcpyAsync(fx, s, b, hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipMemcpyAsync(fy, s + c.g.ncell, b, hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipMemcpyAsync(fz, s + 2 * c.g.ncell, b, hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return MFC_OK;
+}
+
+int mfc_get_forcing_state(mfc_ctx* ctx, int32_t* iseedf, int32_t* iyf, int32_t* ivf, double* b1r,
+                          double* b2r, double* b3r, double* a1r, double* a2r, double* a3r) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  if (iseedf) *iseedf = c.iseedf;
+  if (iyf) *iyf = c.iyf;
+  if (ivf) memcpy(ivf, c.ivf, sizeof c.ivf);
+  double* bs[3] = {b1r, b2r, b3r};
+  double* as[3] = {a1r, a2r, a3r};
+  for (int m = 0; m < 3; ++m) {
+    if (bs[m]) memcpy(bs[m], c.bmod[m], sizeof(double) * 150);
+    if (as[m]) memcpy(as[m], c.amod[m], sizeof(double) * 150);
+  }
+  return MFC_OK;
+}
+
+int mfc_set_forcing_state(mfc_ctx* ctx, int32_t iseedf, int32_t iyf, const int32_t* ivf, const double* b1r,
+                          const double* b2r, const double* b3r, const double* a1r, const double* a2r,
+                          const double* a3r) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  c.iseedf = iseedf;
+  c.iyf = iyf;
+  if (ivf) memcpy(c.ivf, ivf, sizeof c.ivf);
+  const double* bs[3] = {b1r, b2r, b3r};
+  const double* as[3] = {a1r, a2r, a3r};
+  for (int m = 0; m < 3; ++m) {
+    if (bs[m]) memcpy(c.bmod[m], bs[m], sizeof(double) * 150);
+    if (as[m]) memcpy(c.amod[m], as[m], sizeof(double) * 150);
+  }
+  if (a1r || a2r || a3r) return publish_modes(ctx);
+  return MFC_OK;
+}
+
+// ---- fluid step ----------------------------------------------------------------
+int mfc_initpop(mfc_ctx* ctx) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  int rc = need_arrays(ctx);
+  if (rc) return rc;
+  HIPCHK(launch_initpop(c));
+  c.f_version++;
+  c.mom = MOM_ARRAYS;
+  return MFC_OK;
+}
+
+int mfc_forcing(mfc_ctx* ctx) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  if (!(c.p.xkf > 0.0)) return MFC_OK;   // no forced shell: the field stays the body force
+  forcing_modes_host(c);
+  return publish_modes(ctx);
+}
+
+int mfc_collision_mrt(mfc_ctx* ctx) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  if (c.phase != PHASE_POST_STREAM) return fail(ctx, MFC_ERR_STATE, "collision_MRT called twice without streaming");
+  bool arrays;
+  if (c.mom == MOM_VIRTUAL && c.mom_version == c.f_version) {
+    arrays = false;
+  } else if (c.mom == MOM_ARRAYS) {
+    arrays = true;
+  } else if (c.mom == MOM_NONE) {
+    int rc = ensure_macro(ctx);
+    if (rc) return rc;
+    c.mom = MOM_ARRAYS;
+    arrays = true;
+  } else {
+    return fail(ctx, MFC_ERR_STATE, "collision_MRT: rho/u from macrovar no longer match the field");
+  }
+  if (c.timing && c.tpool_used + 2 <= (int)c.tpool.size()) {
+    HIPCHK(hipEventRecord(c.tpool[c.tpool_used], c.stream));
+    HIPCHK(launch_collide(c, arrays));
+    HIPCHK(hipEventRecord(c.tpool[c.tpool_used + 1], c.stream));
+    c.tpool_used += 2;
+  } else {
+    HIPCHK(launch_collide(c, arrays));
+  }
+  c.cur ^= 1;
+  c.phase = PHASE_PRE_STREAM;
+  c.f_version++;
+  return MFC_OK;
+}
+
+int mfc_streaming(mfc_ctx* ctx) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  if (c.phase != PHASE_PRE_STREAM) return fail(ctx, MFC_ERR_STATE, "streaming called twice without collision");
+  c.phase = PHASE_POST_STREAM;
+  c.f_version++;
+  return MFC_OK;
+}
+
+int mfc_rhoupdat(mfc_ctx* ctx, double* maxdiff) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  int rc = need_arrays(ctx);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_set_double, dim3(1), dim3(1), 0, c.stream, c.d_red, 0.0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(launch_rhoupdat(c, c.phase == PHASE_POST_STREAM, c.d_red));
+  c.mom = MOM_ARRAYS;
+  if (maxdiff) {
+    HIPCHK(hipMemcpyAsync(c.h_red, c.d_red, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    *maxdiff = c.h_red[0];
+  }
+  return MFC_OK;
+}
+
+int mfc_macrovar(mfc_ctx* ctx) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  if (c.phase != PHASE_POST_STREAM) return fail(ctx, MFC_ERR_STATE, "macrovar expects the streamed field");
+  c.mom = MOM_VIRTUAL;
+  c.mom_version = c.f_version;
+  c.ft.mac = c.ft.cur;
+  return MFC_OK;
+}
+
+// ---- particles -------------------------------------------------------------------
+int mfc_prt_init(mfc_ctx* ctx, int32_t npart) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  Particles& P = c.pt;
+  if (npart < 0) return MFC_ERR_ARG;
+  if (npart > 0) {
+    const int need = 2 * (int)ceil(c.p.rad + 2.0) + 6;
+    if (c.g.nx < need || c.g.nz < need || (!c.g.walls_y && c.g.ny < need))
+      return fail(ctx, MFC_ERR_ARG, "box too small for the particle radius (need > 2*(rad+2)+5 nodes)");
+    if (!(c.p.rad > 0.0)) return fail(ctx, MFC_ERR_ARG, "particle radius must be positive");
+  }
+  HIPCHK(hipStreamSynchronize(c.stream));
+  if (npart > P.np_max) {
+    double** pd[] = {&P.yp, &P.ypglb, &P.ypglbp, &P.wp, &P.wpp, &P.omgp, &P.omgpp, &P.fHIp, &P.flubp,
+                     &P.torqp, &P.forcep, &P.forcepp, &P.torqpp, &P.dwdt, &P.domgdt, &P.thetap,
+                     &P.map_pos[0], &P.map_pos[1]};
+    for (double** a : pd) {
+      if (*a) HIPCHK(hipFree(*a));
+      *a = nullptr;
+      HIPCHK(hipMalloc(a, sizeof(double) * 3 * npart));
+    }
+    if (P.lk_cnt) HIPCHK(hipFree(P.lk_cnt));
+    if (P.sol_cnt) HIPCHK(hipFree(P.sol_cnt));
+    if (P.lk_off) HIPCHK(hipFree(P.lk_off));
+    HIPCHK(hipMalloc(&P.lk_cnt, sizeof(int32_t) * npart));
+    HIPCHK(hipMalloc(&P.sol_cnt, sizeof(int32_t) * npart));
+    HIPCHK(hipMalloc(&P.lk_off, sizeof(int64_t) * (npart + 1)));
+    // link pool: every link's node lies in the shell r < d <= r + sqrt(2)
+    // around its centre, and a node outside a convex sphere sees it along at
+    // most 9 of the 18 directions (never along both c and -c).
+    const double r = c.p.rad, h = sqrt(3.0) / 2.0;
+    const double ro = r + sqrt(2.0) + h, ri = std::max(0.0, r - h);
+    const double shell = 4.0 / 3.0 * M_PI * (ro * ro * ro - ri * ri * ri);
+    const int64_t per = (int64_t)ceil(shell) * 9 + 64;
+    const int64_t cap = per * npart;
+    if (P.lk_node) HIPCHK(hipFree(P.lk_node));
+    if (P.lk_dir) HIPCHK(hipFree(P.lk_dir));
+    if (P.lk_alpha) HIPCHK(hipFree(P.lk_alpha));
+    if (P.lk_out) HIPCHK(hipFree(P.lk_out));
+    HIPCHK(hipMalloc(&P.lk_node, sizeof(uint32_t) * cap));
+    HIPCHK(hipMalloc(&P.lk_dir, sizeof(uint8_t) * cap));
+    HIPCHK(hipMalloc(&P.lk_alpha, sizeof(double) * cap));
+    HIPCHK(hipMalloc(&P.lk_out, sizeof(double) * cap));
+    P.lk_cap = cap;
+    P.np_max = npart;
+  }
+  if (npart > 0 && !P.isn[0]) {
+    const size_t words = (size_t)((c.g.ncell + 31) / 32);
+    for (int b = 0; b < 2; ++b) {
+      HIPCHK(hipMalloc(&P.isn[b], sizeof(int32_t) * c.g.ncell));
+      HIPCHK(hipMalloc(&P.mask[b], sizeof(uint32_t) * words));
+    }
+  }
+  if (P.isn[0]) {
+    const size_t words = (size_t)((c.g.ncell + 31) / 32);
+    for (int b = 0; b < 2; ++b) {
+      HIPCHK(hipMemsetAsync(P.isn[b], 0, sizeof(int32_t) * c.g.ncell, c.stream));
+      HIPCHK(hipMemsetAsync(P.mask[b], 0, sizeof(uint32_t) * words, c.stream));
+      P.map_np[b] = 0;
+    }
+  }
+  P.map_cur = P.map_old = 0;
+  P.np = npart;
+  if (npart > 0) {
+    double* pd[] = {P.yp, P.ypglb, P.ypglbp, P.wp, P.wpp, P.omgp, P.omgpp, P.fHIp, P.flubp, P.torqp,
+                    P.forcep, P.forcepp, P.torqpp, P.dwdt, P.domgdt, P.thetap};
+    for (double* a : pd) HIPCHK(hipMemsetAsync(a, 0, sizeof(double) * 3 * npart, c.stream));
+    HIPCHK(hipMemsetAsync(P.lk_off, 0, sizeof(int64_t) * (npart + 1), c.stream));
+  }
+  HIPCHK(hipMemsetAsync(c.d_status, 0, sizeof(int32_t) * ST_COUNT, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return MFC_OK;
+}
+
+static double* pfield(Particles& P, int f) {
+  switch (f) {
+    case MFC_PF_YPGLB: return P.ypglb;
+    case MFC_PF_YPGLBP: return P.ypglbp;
+    case MFC_PF_WP: return P.wp;
+    case MFC_PF_WPP: return P.wpp;
+    case MFC_PF_OMGP: return P.omgp;
+    case MFC_PF_OMGPP: return P.omgpp;
+    case MFC_PF_FHIP: return P.fHIp;
+    case MFC_PF_FLUBP: return P.flubp;
+    case MFC_PF_TORQP: return P.torqp;
+    case MFC_PF_FORCEPP: return P.forcepp;
+    case MFC_PF_TORQPP: return P.torqpp;
+    case MFC_PF_DWDT: return P.dwdt;
+    case MFC_PF_DOMGDT: return P.domgdt;
+    case MFC_PF_THETAP: return P.thetap;
+    default: return nullptr;
+  }
+}
+
+int mfc_prt_set(mfc_ctx* ctx, int32_t field, const double* a) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  Particles& P = c.pt;
+  double* d = pfield(P, field);
+  if (!d || !a || P.np <= 0) return MFC_ERR_ARG;
+  const size_t b = sizeof(double) * 3 * P.np;
+  HIPCHK(hipMemcpyAsync(d, a, b, hipMemcpyHostToDevice, c.stream));
+  if (field == MFC_PF_YPGLB) HIPCHK(hipMemcpyAsync(P.yp, a, b, hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return MFC_OK;
+}
+
+int mfc_prt_get(mfc_ctx* ctx, int32_t field, double* a) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  Particles& P = c.pt;
+  double* d = pfield(P, field);
+  if (!d || !a || P.np <= 0) return MFC_ERR_ARG;
+  HIPCHK(hipMemcpyAsync(a, d, sizeof(double) * 3 * P.np, hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return MFC_OK;
+}
+
+// initpartpos (partlib.f90:1-80): seeded rejection sampling on the host; only
+// the flagged particles (lower id of each close pair) are redrawn.
+int mfc_initpartpos(mfc_ctx* ctx, int32_t* iseedp, int32_t mingap_check) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  Particles& P = c.pt;
+  if (!iseedp || P.np <= 0) return MFC_ERR_ARG;
+  const int np = P.np;
+  std::vector<double> y(3 * (size_t)np);
+  std::vector<int> flag(np, 1);
+  int32_t idum = -*iseedp, iy = 0, iv[32] = {0};
+  const int ns[3] = {c.g.nx, c.g.ny, c.g.nz};
+  for (int round = 0;; ++round) {
+    for (int p = 0; p < np; ++p)
+      if (flag[p] > 0)
+        for (int k = 0; k < 3; ++k) y[3 * p + k] = (double)ns[k] * ran1(&idum, &iy, iv);
+    int cnt = 0;
+    for (int a = 0; a < np; ++a) {
+      flag[a] = 0;
+      if (a == np - 1) break;
+      const double xc = y[3 * a], yc = y[3 * a + 1], zc = y[3 * a + 2];
+      for (int b = a + 1; b < np; ++b) {
+        const double xb = near_image(y[3 * b], xc, c.g.nx);
+        const double yb = c.g.walls_y ? y[3 * b + 1] : near_image(y[3 * b + 1], yc, c.g.ny);
+        const double zb = near_image(y[3 * b + 2], zc, c.g.nz);
+        const double dist = sqrt((xc - xb) * (xc - xb) + (yc - yb) * (yc - yb) + (zc - zb) * (zc - zb));
+        double hgap = dist - 2.0 * c.p.rad;
+        if (mingap_check) hgap = hgap - c.p.mingap;
+        if (hgap < 0.0) {
+          flag[a] = 1;
+          break;
+        }
+      }
+      cnt += flag[a];
+    }
+    if (cnt == 0) break;
+    if (round > 100000) return fail(ctx, MFC_ERR_CAPACITY, "initpartpos: no non-overlapping placement found");
+  }
+  *iseedp = idum;
+  HIPCHK(hipMemcpyAsync(P.ypglb, y.data(), sizeof(double) * y.size(), hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipMemcpyAsync(P.yp, y.data(), sizeof(double) * y.size(), hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return MFC_OK;
+}
+
+int mfc_initpartvel(mfc_ctx* ctx) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  if (c.pt.np <= 0) return MFC_ERR_ARG;
+  int rc = need_arrays(ctx);
+  if (rc) return rc;
+  HIPCHK(launch_initpartvel(c));
+  return MFC_OK;
+}
+
+int mfc_initpartomg(mfc_ctx* ctx) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  if (c.pt.np <= 0) return MFC_ERR_ARG;
+  HIPCHK(hipMemsetAsync(c.pt.omgp, 0, sizeof(double) * 3 * c.pt.np, c.stream));
+  return MFC_OK;
+}
+
+int mfc_beads_links(mfc_ctx* ctx) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  Particles& P = c.pt;
+  if (P.np <= 0) return MFC_ERR_ARG;
+  const int target = (P.map_cur != P.map_old) ? P.map_cur : (P.map_cur ^ 1);
+  HIPCHK(set_status(c, ST_ALPHA_FAULT, 0));
+  HIPCHK(launch_clear_map(c, target));
+  HIPCHK(launch_links(c, target));
+  P.map_cur = target;
+  return MFC_OK;
+}
+
+int mfc_beads_collision(mfc_ctx* ctx) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  if (c.pt.np <= 0 || c.pt.map_np[c.pt.map_cur] == 0) return fail(ctx, MFC_ERR_STATE, "beads_collision before beads_links");
+  if (c.phase != PHASE_PRE_STREAM) return fail(ctx, MFC_ERR_STATE, "beads_collision expects the post-collision field");
+  HIPCHK(set_status(c, ST_CONTACT_LINKS, 0));
+  HIPCHK(launch_bounce_back(c));
+  c.f_version++;
+  return MFC_OK;
+}
+
+int mfc_beads_lubforce(mfc_ctx* ctx) {
+  NEEDCTX;
+  if (ctx->c.pt.np <= 0) return MFC_ERR_ARG;
+  HIPCHK(launch_lubforce(ctx->c));
+  return MFC_OK;
+}
+
+int mfc_beads_move(mfc_ctx* ctx, int32_t istep) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  if (c.pt.np <= 0) return MFC_ERR_ARG;
+  HIPCHK(launch_move(c, istep == c.p.irelease ? 1 : 0));
+  c.pt.map_old = c.pt.map_cur;   // ibnodes0 = ibnodes (partlib.f90:2376-2377)
+  return MFC_OK;
+}
+
+int mfc_beads_redistribute(mfc_ctx* ctx, int32_t istep) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  if (c.pt.np <= 0) return MFC_ERR_ARG;
+  HIPCHK(launch_redistribute(c, istep));
+  return check_faults(ctx);
+}
+
+int mfc_beads_filling(mfc_ctx* ctx) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  if (c.pt.np <= 0) return MFC_ERR_ARG;
+  if (c.phase != PHASE_POST_STREAM) return fail(ctx, MFC_ERR_STATE, "beads_filling expects the streamed field");
+  HIPCHK(set_status(c, ST_REFILL, 0));
+  HIPCHK(launch_filling(c));
+  c.f_version++;
+  return MFC_OK;
+}
+
+int mfc_get_maps(mfc_ctx* ctx, int32_t which, int32_t* ib, int32_t* is) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  if (!ib || !is) return MFC_ERR_ARG;
+  const size_t b = sizeof(int32_t) * c.g.ncell;
+  int rc = ensure_staging(ctx, 2 * b);
+  if (rc) return rc;
+  int32_t* s = (int32_t*)c.staging;
+  HIPCHK(launch_maps_to_host_layout(c, which, s, s + c.g.ncell));
+  HIPCHK(hipMemcpyAsync(ib, s, b, hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipMemcpyAsync(is, s + c.g.ncell, b, hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return MFC_OK;
+}
+
+int mfc_get_links(mfc_ctx* ctx, int32_t* ilink, int32_t* mlink, double* alink) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  if (!ilink || !mlink || !alink) return MFC_ERR_ARG;
+  if (c.g.ncell > (1u << 21)) return fail(ctx, MFC_ERR_ARG, "mfc_get_links: dense link arrays limited to 2^21 nodes");
+  if (c.pt.np <= 0) return MFC_ERR_ARG;
+  const uint64_t n18 = 18ull * c.g.ncell;
+  int rc = ensure_staging(ctx, n18 * (4 + 4 + 8));
+  if (rc) return rc;
+  int32_t* il = (int32_t*)c.staging;
+  int32_t* ml = il + n18;
+  double* al = (double*)(ml + n18);
+  const unsigned nb = (unsigned)((n18 + 255) / 256);
+  hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, c.stream, il, n18, -1);
+  hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, c.stream, ml, n18, -1);
+  hipLaunchKernelGGL(k_fill_f64, dim3(nb), dim3(256), 0, c.stream, al, n18, 0.0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(launch_links_dense(c, il, ml, al));
+  HIPCHK(hipMemcpyAsync(ilink, il, n18 * 4, hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipMemcpyAsync(mlink, ml, n18 * 4, hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipMemcpyAsync(alink, al, n18 * 8, hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return MFC_OK;
+}
+
+// ---- whole step ---------------------------------------------------------------
+int mfc_step(mfc_ctx* ctx, int32_t istep, int32_t nsteps) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  int rc;
+  for (int s = 0; s < nsteps; ++s) {
+    const int it = istep + s;
+    const bool on = particles_on(c) && it >= c.p.irelease;
+    if ((rc = mfc_forcing(ctx))) return rc;
+    if ((rc = mfc_collision_mrt(ctx))) return rc;
+    if (on && (rc = mfc_beads_collision(ctx))) return rc;
+    if ((rc = mfc_streaming(ctx))) return rc;
+    if (on) {
+      if ((rc = mfc_beads_lubforce(ctx))) return rc;
+      if ((rc = mfc_beads_move(ctx, it))) return rc;
+      HIPCHK(launch_redistribute(c, it));   // overlap reported by the next mfc_sync
+      if ((rc = mfc_beads_links(ctx))) return rc;
+      if ((rc = mfc_beads_filling(ctx))) return rc;
+    }
+    if ((rc = mfc_macrovar(ctx))) return rc;
+  }
+  return MFC_OK;
+}
+
+int mfc_set_timing(mfc_ctx* ctx, int32_t on) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  if (on && c.tpool.empty()) {
+    c.tpool.resize(kTimingPool);
+    for (auto& e : c.tpool) HIPCHK(hipEventCreate(&e));
+  }
+  HIPCHK(hipStreamSynchronize(c.stream));
+  c.tpool_used = 0;
+  c.timing = on ? 1 : 0;
+  return MFC_OK;
+}
+
+int mfc_get_timing(mfc_ctx* ctx, double* total_ms, int64_t* launches) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  HIPCHK(hipStreamSynchronize(c.stream));
+  double t = 0.0;
+  for (int i = 0; i + 1 < c.tpool_used; i += 2) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c.tpool[i], c.tpool[i + 1]));
+    t += ms;
+  }
+  if (total_ms) *total_ms = t;
+  if (launches) *launches = c.tpool_used / 2;
+  return MFC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,465 @@
+// fluid.hip -- lattice kernels of libmfc_lbm for gfx950.
+//
+// Storage: populations are SoA planes pop[i][n], n = x + nx*(y + ny*z), in
+// FP64 or FP32 (deviations f_i - w_i*rho0).  The resident buffer holds the
+// post-collision (pre-streaming) field; streaming is never a separate pass:
+// every consumer pulls f_i(x) = buf[i][x - c_i] (half-way bounce-back at the
+// y walls when they are on).  One kernel therefore streams, takes moments,
+// evaluates the forcing field and collides, reading 19 and writing 19
+// populations per node: the 152 B (FP32) / 304 B (FP64) per node that bound
+// the step.  Each lane owns one node; consecutive lanes are consecutive x,
+// so every plane access of a wavefront is a contiguous 256 B (FP32) or
+// 512 B (FP64) run.
+#include "internal.h"
+#include "lbm_access.h"
+#include "lbm_collide.h"
+
+namespace mfc {
+
+struct ForceArgs {
+  const double* gcur;   // forcing table of the current step (may be null)
+  const double* gmac;   // forcing table captured by the last macrovar (may be null)
+  const double* phx;    // (cos,sin)(2 pi m / nx)
+  double bfx, bfy, bfz; // uniform body force
+};
+
+// Forcing field at node x of row `row` from a per-row table
+//   F_c = Re G_c0 + 2 Re(e1 G_c1) + 2 Re(e2 G_c2),  e_k = exp(i 2 pi k x / nx),
+// the direct form of the reference's inverse real FFT of its few modes
+// (gaussian.f90:325-369, mpifft.f90:111-158).
+__device__ __forceinline__ void force_at(const double* __restrict__ G, const double* __restrict__ phx,
+                                         uint32_t row, int x, int nx, double& Fx, double& Fy,
+                                         double& Fz) {
+  const double* t = G + 18ull * row;
+  const int x2 = (2 * x >= nx) ? 2 * x - nx : 2 * x;
+  const double c1 = phx[2 * x], s1 = phx[2 * x + 1];
+  const double c2 = phx[2 * x2], s2 = phx[2 * x2 + 1];
+  double v[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double* q = t + 6 * c;
+    v[c] = q[0] + 2.0 * (q[2] * c1 - q[3] * s1) + 2.0 * (q[4] * c2 - q[5] * s2);
+  }
+  Fx = v[0];
+  Fy = v[1];
+  Fz = v[2];
+}
+
+// Fused stream + macro + forcing + MRT collision.  ARR: conserved moments
+// from the rho/u arrays (the reference's literal data flow, used while they
+// are authoritative, e.g. the pre-relaxation); otherwise recomputed from the
+// pulled populations plus half the forcing captured at the last macrovar,
+// which is exactly what those arrays would hold.
+template <typename T, typename R, bool ARR, bool WALLS>
+__global__ void __launch_bounds__(256)
+    k_collide(const T* __restrict__ A, T* __restrict__ B, const uint32_t* __restrict__ mask, Grid g,
+              CollideConsts k, ForceArgs fa, const double* __restrict__ rho_a,
+              const double* __restrict__ ux_a, const double* __restrict__ uy_a,
+              const double* __restrict__ uz_a) {
+  const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+  if (n >= g.ncell) return;
+  int x, y, z;
+  uint32_t row;
+  node_coords(n, g, x, y, z, row);
+  R f[NPOP];
+  load_pulled<T, R, WALLS>(A, g, x, y, z, n, f);
+  const bool solid = mask != nullptr && ((mask[n >> 5] >> (n & 31u)) & 1u);
+  if (!solid) {
+    R rho, ux, uy, uz;
+    if (ARR) {
+      rho = (R)rho_a[n];
+      ux = (R)ux_a[n];
+      uy = (R)uy_a[n];
+      uz = (R)uz_a[n];
+    } else {
+      R jx, jy, jz;
+      density_momentum<R>(f, rho, jx, jy, jz);
+      double mx = fa.bfx, my = fa.bfy, mz = fa.bfz;
+      if (fa.gmac) {
+        double gx, gy, gz;
+        force_at(fa.gmac, fa.phx, row, x, g.nx, gx, gy, gz);
+        mx += gx;
+        my += gy;
+        mz += gz;
+      }
+      ux = jx + (R)(0.5 * mx);
+      uy = jy + (R)(0.5 * my);
+      uz = jz + (R)(0.5 * mz);
+    }
+    double Fx = fa.bfx, Fy = fa.bfy, Fz = fa.bfz;
+    if (fa.gcur) {
+      double gx, gy, gz;
+      force_at(fa.gcur, fa.phx, row, x, g.nx, gx, gy, gz);
+      Fx += gx;
+      Fy += gy;
+      Fz += gz;
+    }
+    mrt_collide<R>(f, rho, ux, uy, uz, (R)Fx, (R)Fy, (R)Fz, k);
+  }
+#pragma unroll
+  for (int i = 0; i < NPOP; ++i) B[(uint64_t)i * g.ncell + n] = (T)f[i];
+}
+
+// Forcing table of one step: per (y,z) row the nine partial sums
+//   G_{c,kx}(y,z) = sum_{ky,kz} A_c(kx,ky,kz) exp(i 2 pi (ky y/ny + kz z/nz))
+// over the reference's 3 x 5 x 5 low modes (gaussian.f90:243-359).
+__global__ void k_force_table(const double* __restrict__ modes, const double* __restrict__ phy,
+                              const double* __restrict__ phz, int ny, int nz,
+                              double* __restrict__ G) {
+  const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= (uint32_t)ny * (uint32_t)nz) return;
+  const int y = (int)(row % (uint32_t)ny), z = (int)(row / (uint32_t)ny);
+  for (int c = 0; c < 3; ++c) {
+    const double* a = modes + 150 * c;
+    for (int kx = 0; kx < 3; ++kx) {
+      double gr = 0.0, gi = 0.0;
+      for (int k = 1; k <= 5; ++k) {
+        const int kz = k <= 3 ? k - 1 : k - 6;
+        const int mz = wrapi((kz * z) % nz, nz);
+        const double ezr = phz[2 * mz], ezi = phz[2 * mz + 1];
+        for (int j = 1; j <= 5; ++j) {
+          const int ky = j <= 3 ? j - 1 : j - 6;
+          const int my = wrapi((ky * y) % ny, ny);
+          const double eyr = phy[2 * my], eyi = phy[2 * my + 1];
+          const int ai = 2 * kx + 6 * ((j - 1) + 5 * (k - 1));
+          const double ar = a[ai], aim = a[ai + 1];
+          const double er = eyr * ezr - eyi * ezi, ei = eyr * ezi + eyi * ezr;
+          gr += ar * er - aim * ei;
+          gi += ar * ei + aim * er;
+        }
+      }
+      G[18ull * row + 6 * c + 2 * kx] = gr;
+      G[18ull * row + 6 * c + 2 * kx + 1] = gi;
+    }
+  }
+}
+
+// macrovar (collision.f90:232-332): rho = sum f, u = sum c f + F/2, then the
+// rigid-body velocity and rho = rhopart at solid nodes.
+template <typename T, bool PULLED, bool WALLS>
+__global__ void __launch_bounds__(256)
+    k_macro(const T* __restrict__ A, Grid g, ForceArgs fa, const uint32_t* __restrict__ mask,
+            const int32_t* __restrict__ isn, const double* __restrict__ ypglb,
+            const double* __restrict__ wp, const double* __restrict__ omgp, double rhopart,
+            double* __restrict__ rho, double* __restrict__ ux, double* __restrict__ uy,
+            double* __restrict__ uz) {
+  const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+  if (n >= g.ncell) return;
+  int x, y, z;
+  uint32_t row;
+  node_coords(n, g, x, y, z, row);
+  double f[NPOP];
+  if (PULLED)
+    load_pulled<T, double, WALLS>(A, g, x, y, z, n, f);
+  else
+    load_direct<T, double>(A, g, n, f);
+  double r, jx, jy, jz;
+  density_momentum<double>(f, r, jx, jy, jz);
+  double Fx = fa.bfx, Fy = fa.bfy, Fz = fa.bfz;
+  if (fa.gmac) {
+    double gx, gy, gz;
+    force_at(fa.gmac, fa.phx, row, x, g.nx, gx, gy, gz);
+    Fx += gx;
+    Fy += gy;
+    Fz += gz;
+  }
+  double u = jx + Fx / 2.0, v = jy + Fy / 2.0, w = jz + Fz / 2.0;
+  if (mask != nullptr && ((mask[n >> 5] >> (n & 31u)) & 1u)) {
+    const int id = isn[n] - 1;
+    const double xp = (double)(x + 1) - 0.5, yp = (double)(y + 1) - 0.5, zp = (double)(z + 1) - 0.5;
+    const double xc = near_image(ypglb[3 * id], xp, g.nx);
+    const double yc = WALLS ? ypglb[3 * id + 1] : near_image(ypglb[3 * id + 1], yp, g.ny);
+    const double zc = near_image(ypglb[3 * id + 2], zp, g.nz);
+    const double xx = xp - xc, yy = yp - yc, zz = zp - zc;
+    const double* W = wp + 3 * id;
+    const double* O = omgp + 3 * id;
+    u = W[0] + (O[1] * zz - O[2] * yy);
+    v = W[1] + (O[2] * xx - O[0] * zz);
+    w = W[2] + (O[0] * yy - O[1] * xx);
+    r = rhopart;
+  }
+  rho[n] = r;
+  ux[n] = u;
+  uy[n] = v;
+  uz[n] = w;
+}
+
+// rhoupdat (collision.f90:213-229) plus the pre-relaxation residual
+// max|rho_new - rho_old| (main.f90:91-96).
+template <typename T, bool PULLED, bool WALLS>
+__global__ void __launch_bounds__(256)
+    k_rhoupdat(const T* __restrict__ A, Grid g, double* __restrict__ rho,
+               unsigned long long* __restrict__ maxbits) {
+  __shared__ double red[256];
+  const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+  double d = 0.0;
+  if (n < g.ncell) {
+    int x, y, z;
+    uint32_t row;
+    node_coords(n, g, x, y, z, row);
+    double f[NPOP];
+    if (PULLED)
+      load_pulled<T, double, WALLS>(A, g, x, y, z, n, f);
+    else
+      load_direct<T, double>(A, g, n, f);
+    double r = f[0];
+#pragma unroll
+    for (int i = 1; i < NPOP; ++i) r = r + f[i];
+    d = fabs(r - rho[n]);
+    rho[n] = r;
+  }
+  red[threadIdx.x] = d;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && maxbits) atomicMax(maxbits, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+// initpop (initial.f90:3-28): deviation equilibrium with rho = 0.
+template <typename T, bool WALLS>
+__global__ void __launch_bounds__(256)
+    k_initpop(T* __restrict__ A, Grid g, CollideConsts k, bool pulled, double* __restrict__ rho,
+              const double* __restrict__ ux, const double* __restrict__ uy,
+              const double* __restrict__ uz) {
+  const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+  if (n >= g.ncell) return;
+  int x, y, z;
+  uint32_t row;
+  node_coords(n, g, x, y, z, row);
+  rho[n] = 0.0;
+  const double u = ux[n], v = uy[n], w = uz[n];
+  const double usqr = 1.5 * (u * u + v * v + w * w);
+#pragma unroll
+  for (int i = 0; i < NPOP; ++i) {
+    const double G = (double)cx(i) * u + (double)cy(i) * v + (double)cz(i) * w;
+    const double wt = i == 0 ? k.ww0 : (i <= 6 ? k.ww1 : k.ww2);
+    const double fe = i == 0 ? wt * (0.0 - usqr) : wt * (0.0 + 3.0 * G + 4.5 * G * G - usqr);
+    store_phase<T, WALLS>(A, g, i, x, y, z, n, pulled, fe);
+  }
+}
+
+// SoA (resident, current phase) -> AoS double f(0:18, ...) for z in [z0, z0+nzs)
+template <typename T, bool WALLS>
+__global__ void __launch_bounds__(256)
+    k_pop_to_aos(const T* __restrict__ A, Grid g, double* __restrict__ dst, int z0, uint32_t count,
+                 bool pulled) {
+  const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+  if (q >= count) return;
+  const uint32_t n = q + (uint32_t)z0 * (uint32_t)g.nx * (uint32_t)g.ny;
+  int x, y, z;
+  uint32_t row;
+  node_coords(n, g, x, y, z, row);
+  double f[NPOP];
+  if (pulled)
+    load_pulled<T, double, WALLS>(A, g, x, y, z, n, f);
+  else
+    load_direct<T, double>(A, g, n, f);
+#pragma unroll
+  for (int i = 0; i < NPOP; ++i) dst[19ull * q + i] = f[i];
+}
+
+template <typename T, bool WALLS>
+__global__ void __launch_bounds__(256)
+    k_aos_to_pop(T* __restrict__ A, Grid g, const double* __restrict__ src, int z0, uint32_t count,
+                 bool pulled) {
+  const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+  if (q >= count) return;
+  const uint32_t n = q + (uint32_t)z0 * (uint32_t)g.nx * (uint32_t)g.ny;
+  int x, y, z;
+  uint32_t row;
+  node_coords(n, g, x, y, z, row);
+#pragma unroll
+  for (int i = 0; i < NPOP; ++i) store_phase<T, WALLS>(A, g, i, x, y, z, n, pulled, src[19ull * q + i]);
+}
+
+__global__ void __launch_bounds__(256)
+    k_force_field(Grid g, ForceArgs fa, double* __restrict__ fx, double* __restrict__ fy,
+                  double* __restrict__ fz) {
+  const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+  if (n >= g.ncell) return;
+  int x, y, z;
+  uint32_t row;
+  node_coords(n, g, x, y, z, row);
+  double Fx = fa.bfx, Fy = fa.bfy, Fz = fa.bfz;
+  if (fa.gcur) {
+    double gx, gy, gz;
+    force_at(fa.gcur, fa.phx, row, x, g.nx, gx, gy, gz);
+    Fx += gx;
+    Fy += gy;
+    Fz += gz;
+  }
+  fx[n] = Fx;
+  fy[n] = Fy;
+  fz[n] = Fz;
+}
+
+// ---------------------------------------------------------------------------
+static inline unsigned nblocks(uint64_t n) { return (unsigned)((n + 255) / 256); }
+
+static ForceArgs force_args(const Ctx& c) {
+  ForceArgs fa;
+  fa.gcur = c.ft.live[c.ft.cur] ? c.ft.g[c.ft.cur] : nullptr;
+  fa.gmac = c.ft.live[c.ft.mac] ? c.ft.g[c.ft.mac] : nullptr;
+  fa.phx = c.ft.phx;
+  fa.bfx = c.p.body_force[0];
+  fa.bfy = c.p.body_force[1];
+  fa.bfz = c.p.body_force[2];
+  return fa;
+}
+
+hipError_t launch_force_table(Ctx& c, int slot) {
+  const uint32_t rows = (uint32_t)c.g.ny * (uint32_t)c.g.nz;
+  hipLaunchKernelGGL(k_force_table, dim3((rows + 127) / 128), dim3(128), 0, c.stream, c.ft.d_modes,
+                     c.ft.phy, c.ft.phz, c.g.ny, c.g.nz, c.ft.g[slot]);
+  return hipGetLastError();
+}
+
+template <typename T, typename R>
+static void collide_dispatch(Ctx& c, bool arrays, const uint32_t* mask, ForceArgs fa) {
+  const T* A = (const T*)c.pop[c.cur];
+  T* B = (T*)c.pop[c.cur ^ 1];
+  const dim3 grid(nblocks(c.g.ncell)), blk(256);
+  if (arrays) {
+    if (c.g.walls_y)
+      hipLaunchKernelGGL((k_collide<T, R, true, true>), grid, blk, 0, c.stream, A, B, mask, c.g, c.cc,
+                         fa, c.rho, c.ux, c.uy, c.uz);
+    else
+      hipLaunchKernelGGL((k_collide<T, R, true, false>), grid, blk, 0, c.stream, A, B, mask, c.g, c.cc,
+                         fa, c.rho, c.ux, c.uy, c.uz);
+  } else {
+    if (c.g.walls_y)
+      hipLaunchKernelGGL((k_collide<T, R, false, true>), grid, blk, 0, c.stream, A, B, mask, c.g,
+                         c.cc, fa, nullptr, nullptr, nullptr, nullptr);
+    else
+      hipLaunchKernelGGL((k_collide<T, R, false, false>), grid, blk, 0, c.stream, A, B, mask, c.g,
+                         c.cc, fa, nullptr, nullptr, nullptr, nullptr);
+  }
+}
+
+hipError_t launch_collide(Ctx& c, bool arrays) {
+  const uint32_t* mask = (c.pt.np > 0 && c.pt.map_np[c.pt.map_cur] > 0) ? c.pt.mask[c.pt.map_cur] : nullptr;
+  ForceArgs fa = force_args(c);
+  if (c.prec == MFC_PREC_FP64)
+    collide_dispatch<double, double>(c, arrays, mask, fa);
+  else
+    collide_dispatch<float, float>(c, arrays, mask, fa);
+  return hipGetLastError();
+}
+
+template <typename T>
+static void macro_dispatch(Ctx& c, bool pulled, ForceArgs fa) {
+  const T* A = (const T*)c.pop[c.cur];
+  const bool parts = c.pt.np > 0 && c.pt.map_np[c.pt.map_cur] > 0;
+  const uint32_t* mask = parts ? c.pt.mask[c.pt.map_cur] : nullptr;
+  const int32_t* isn = parts ? c.pt.isn[c.pt.map_cur] : nullptr;
+  const dim3 grid(nblocks(c.g.ncell)), blk(256);
+#define MFC_MACRO(P, W)                                                                         \
+  hipLaunchKernelGGL((k_macro<T, P, W>), grid, blk, 0, c.stream, A, c.g, fa, mask, isn,         \
+                     c.pt.ypglb, c.pt.wp, c.pt.omgp, c.p.rhopart, c.rho, c.ux, c.uy, c.uz)
+  if (pulled) {
+    if (c.g.walls_y) MFC_MACRO(true, true); else MFC_MACRO(true, false);
+  } else {
+    if (c.g.walls_y) MFC_MACRO(false, true); else MFC_MACRO(false, false);
+  }
+#undef MFC_MACRO
+}
+
+hipError_t launch_macro(Ctx& c, bool pulled) {
+  ForceArgs fa = force_args(c);
+  if (c.prec == MFC_PREC_FP64)
+    macro_dispatch<double>(c, pulled, fa);
+  else
+    macro_dispatch<float>(c, pulled, fa);
+  return hipGetLastError();
+}
+
+template <typename T>
+static void rhoupdat_dispatch(Ctx& c, bool pulled, unsigned long long* mb) {
+  const T* A = (const T*)c.pop[c.cur];
+  const dim3 grid(nblocks(c.g.ncell)), blk(256);
+  if (pulled) {
+    if (c.g.walls_y)
+      hipLaunchKernelGGL((k_rhoupdat<T, true, true>), grid, blk, 0, c.stream, A, c.g, c.rho, mb);
+    else
+      hipLaunchKernelGGL((k_rhoupdat<T, true, false>), grid, blk, 0, c.stream, A, c.g, c.rho, mb);
+  } else {
+    hipLaunchKernelGGL((k_rhoupdat<T, false, false>), grid, blk, 0, c.stream, A, c.g, c.rho, mb);
+  }
+}
+
+hipError_t launch_rhoupdat(Ctx& c, bool pulled, double* d_maxdiff) {
+  unsigned long long* mb = (unsigned long long*)d_maxdiff;
+  if (c.prec == MFC_PREC_FP64)
+    rhoupdat_dispatch<double>(c, pulled, mb);
+  else
+    rhoupdat_dispatch<float>(c, pulled, mb);
+  return hipGetLastError();
+}
+
+hipError_t launch_initpop(Ctx& c) {
+  const dim3 grid(nblocks(c.g.ncell)), blk(256);
+  const bool pulled = c.phase == PHASE_POST_STREAM;
+  if (c.prec == MFC_PREC_FP64) {
+    double* A = (double*)c.pop[c.cur];
+    if (c.g.walls_y)
+      hipLaunchKernelGGL((k_initpop<double, true>), grid, blk, 0, c.stream, A, c.g, c.cc, pulled, c.rho, c.ux, c.uy, c.uz);
+    else
+      hipLaunchKernelGGL((k_initpop<double, false>), grid, blk, 0, c.stream, A, c.g, c.cc, pulled, c.rho, c.ux, c.uy, c.uz);
+  } else {
+    float* A = (float*)c.pop[c.cur];
+    if (c.g.walls_y)
+      hipLaunchKernelGGL((k_initpop<float, true>), grid, blk, 0, c.stream, A, c.g, c.cc, pulled, c.rho, c.ux, c.uy, c.uz);
+    else
+      hipLaunchKernelGGL((k_initpop<float, false>), grid, blk, 0, c.stream, A, c.g, c.cc, pulled, c.rho, c.ux, c.uy, c.uz);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_pop_to_aos(Ctx& c, double* dst, int z0, int nzs, bool pulled) {
+  const uint32_t count = (uint32_t)c.g.nx * (uint32_t)c.g.ny * (uint32_t)nzs;
+  const dim3 grid(nblocks(count)), blk(256);
+  if (c.prec == MFC_PREC_FP64) {
+    const double* A = (const double*)c.pop[c.cur];
+    if (c.g.walls_y)
+      hipLaunchKernelGGL((k_pop_to_aos<double, true>), grid, blk, 0, c.stream, A, c.g, dst, z0, count, pulled);
+    else
+      hipLaunchKernelGGL((k_pop_to_aos<double, false>), grid, blk, 0, c.stream, A, c.g, dst, z0, count, pulled);
+  } else {
+    const float* A = (const float*)c.pop[c.cur];
+    if (c.g.walls_y)
+      hipLaunchKernelGGL((k_pop_to_aos<float, true>), grid, blk, 0, c.stream, A, c.g, dst, z0, count, pulled);
+    else
+      hipLaunchKernelGGL((k_pop_to_aos<float, false>), grid, blk, 0, c.stream, A, c.g, dst, z0, count, pulled);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_aos_to_pop(Ctx& c, const double* src, int z0, int nzs, bool pulled) {
+  const uint32_t count = (uint32_t)c.g.nx * (uint32_t)c.g.ny * (uint32_t)nzs;
+  const dim3 grid(nblocks(count)), blk(256);
+  if (c.prec == MFC_PREC_FP64) {
+    double* A = (double*)c.pop[c.cur];
+    if (c.g.walls_y)
+      hipLaunchKernelGGL((k_aos_to_pop<double, true>), grid, blk, 0, c.stream, A, c.g, src, z0, count, pulled);
+    else
+      hipLaunchKernelGGL((k_aos_to_pop<double, false>), grid, blk, 0, c.stream, A, c.g, src, z0, count, pulled);
+  } else {
+    float* A = (float*)c.pop[c.cur];
+    if (c.g.walls_y)
+      hipLaunchKernelGGL((k_aos_to_pop<float, true>), grid, blk, 0, c.stream, A, c.g, src, z0, count, pulled);
+    else
+      hipLaunchKernelGGL((k_aos_to_pop<float, false>), grid, blk, 0, c.stream, A, c.g, src, z0, count, pulled);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_force_field(Ctx& c, int slot, double* fx, double* fy, double* fz) {
+  ForceArgs fa = force_args(c);
+  fa.gcur = c.ft.live[slot] ? c.ft.g[slot] : nullptr;
+  hipLaunchKernelGGL(k_force_field, dim3(nblocks(c.g.ncell)), dim3(256), 0, c.stream, c.g, fa, fx, fy, fz);
+  return hipGetLastError();
+}
+
+}  // namespace mfc

@@ -1,0 +1,129 @@
+// internal.h -- device-state context and kernel launchers of libmfc_lbm.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/mfc_lbm.h"
+#include "lbm_common.h"
+
+namespace mfc {
+
+enum Phase { PHASE_PRE_STREAM = 0, PHASE_POST_STREAM = 1 };
+enum MomState { MOM_NONE = 0, MOM_ARRAYS = 1, MOM_VIRTUAL = 2 };
+
+// device status words (int32), read back by mfc_sync / mfc_status
+enum StatusWord {
+  ST_OVERLAP = 0,        // particles flagged by the last overlap check
+  ST_OVERLAP_STEP = 1,   // first step number at which an overlap was seen (0: none)
+  ST_LINK_OVERFLOW = 2,  // links that did not fit the link pool
+  ST_ALPHA_FAULT = 3,    // wall fractions outside [0,1] (reported, not fatal)
+  ST_CONTACT_LINKS = 4,  // boundary links whose own node is solid (near contact)
+  ST_NONFINITE = 5,      // particles with non-finite or out-of-box state
+  ST_REFILL = 6,         // nodes refilled by the last beads_filling
+  ST_NLINKS = 7,         // links built by the last beads_links
+  ST_NSOLID = 8,         // solid nodes marked by the last beads_links
+  ST_BOX_FAULT = 9,      // particle boxes that did not fit the lattice
+  ST_COUNT = 16
+};
+
+struct ForceTables {
+  double* g[3] = {nullptr, nullptr, nullptr};  // per (y,z) row: 3 comps x 3 kx x (re,im)
+  int live[3] = {0, 0, 0};                     // 1: table holds nonzero modes
+  int cur = 0;                                 // modes of the last FORCING
+  int mac = 1;                                 // modes captured by the last macrovar
+  double* phx = nullptr;                       // (cos,sin)(2 pi m / nx)
+  double* phy = nullptr;
+  double* phz = nullptr;
+  double* d_modes = nullptr;                   // 3 x 150 mode coefficients
+};
+
+struct Particles {
+  int np = 0, np_max = 0;
+  // (3, np) column-major, c + 3*p
+  double *yp = nullptr, *ypglb = nullptr, *ypglbp = nullptr;
+  double *wp = nullptr, *wpp = nullptr, *omgp = nullptr, *omgpp = nullptr;
+  double *fHIp = nullptr, *flubp = nullptr, *torqp = nullptr;
+  double *forcep = nullptr, *forcepp = nullptr, *torqpp = nullptr;
+  double *dwdt = nullptr, *domgdt = nullptr, *thetap = nullptr;
+  // node maps, double-buffered: isn = particle id (1-based) or 0, mask = solid bit
+  int32_t* isn[2] = {nullptr, nullptr};
+  uint32_t* mask[2] = {nullptr, nullptr};
+  double* map_pos[2] = {nullptr, nullptr};   // centres each map was built from
+  int map_np[2] = {0, 0};                    // particle count of that build (0: map empty)
+  int map_cur = 0, map_old = 0;
+  // boundary links, grouped by particle
+  int64_t lk_cap = 0;
+  uint32_t* lk_node = nullptr;
+  uint8_t* lk_dir = nullptr;
+  double* lk_alpha = nullptr;
+  double* lk_out = nullptr;
+  int32_t* lk_cnt = nullptr;   // per particle
+  int64_t* lk_off = nullptr;   // per particle (np + 1)
+  int32_t* sol_cnt = nullptr;  // solid nodes per particle
+  double* scratch = nullptr;   // reduction scratch
+};
+
+struct Ctx {
+  mfc_params p{};
+  Grid g{};
+  CollideConsts cc{};
+  int prec = 0;                 // 0: FP64 populations, 1: FP32
+  size_t tsize = 8;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  void* pop[2] = {nullptr, nullptr};
+  int cur = 0;
+  int phase = PHASE_POST_STREAM;
+  uint64_t f_version = 1;
+  // macroscopic arrays (reference rho/ux/uy/uz), FP64
+  double *rho = nullptr, *ux = nullptr, *uy = nullptr, *uz = nullptr;
+  int mom = MOM_NONE;
+  uint64_t mom_version = 0;
+  ForceTables ft;
+  // host forcing state (gaussian.f90 FORCING)
+  int32_t iseedf = 0, iyf = 0, ivf[32] = {0};
+  double amod[3][150] = {{0}}, bmod[3][150] = {{0}};
+  double* h_modes = nullptr;        // pinned ring of mode uploads
+  hipEvent_t mode_ev[8] = {};
+  int mode_slot = 0;
+  Particles pt;
+  int32_t* d_status = nullptr;
+  int32_t* h_status = nullptr;      // pinned
+  double* d_red = nullptr;          // small reduction buffer
+  double* h_red = nullptr;          // pinned
+  void* staging = nullptr;          // device staging for host transfers
+  size_t staging_bytes = 0;
+  std::string err;
+  // optional per-launch timing of the collide kernel: event pairs recorded
+  // around each launch, summed by mfc_get_timing
+  int timing = 0;
+  std::vector<hipEvent_t> tpool;
+  int tpool_used = 0;
+};
+
+// ---- launchers (fluid.hip) ----
+hipError_t launch_force_table(Ctx& c, int slot);
+hipError_t launch_collide(Ctx& c, bool arrays);
+hipError_t launch_macro(Ctx& c, bool pulled);
+hipError_t launch_rhoupdat(Ctx& c, bool pulled, double* d_maxdiff);
+hipError_t launch_initpop(Ctx& c);
+hipError_t launch_pop_to_aos(Ctx& c, double* dst, int z0, int nzs, bool pulled);
+hipError_t launch_aos_to_pop(Ctx& c, const double* src, int z0, int nzs, bool pulled);
+hipError_t launch_force_field(Ctx& c, int slot, double* fx, double* fy, double* fz);
+
+// ---- launchers (particles.hip) ----
+hipError_t launch_links(Ctx& c, int target);
+hipError_t launch_clear_map(Ctx& c, int target);
+hipError_t launch_bounce_back(Ctx& c);
+hipError_t launch_lubforce(Ctx& c);
+hipError_t launch_move(Ctx& c, int first);
+hipError_t launch_redistribute(Ctx& c, int istep);
+hipError_t launch_filling(Ctx& c);
+hipError_t launch_initpartvel(Ctx& c);
+hipError_t launch_maps_to_host_layout(Ctx& c, int which, int32_t* ib, int32_t* is);
+hipError_t launch_links_dense(Ctx& c, int32_t* ilink, int32_t* mlink, double* alink);
+
+}  // namespace mfc

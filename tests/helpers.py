@@ -1,0 +1,84 @@
+"""Shared set-up for the parity tests: the same synthetic state on the GPU
+library and on the C oracle."""
+from __future__ import annotations
+
+import numpy as np
+
+from multiphase_flow_codes_amd.params import LBMParams
+
+
+def small_params(n=(20, 24, 16), npart=0, rad=3.0, forcing=True, walls=False, visc=0.01, **kw):
+    nx, ny, nz = n
+    prm = LBMParams(nx=nx, ny=ny, nz=nz, visc=visc, npart=npart, rad=rad, forcing=forcing,
+                    walls_y=walls, **kw)
+    # strong forcing so the forced modes matter within a few steps
+    prm.vscale = 0.05
+    prm.derive()
+    return prm
+
+
+def smooth_velocity(prm, rng, amp=0.02, kmax=2):
+    """A smooth periodic velocity field from a few random Fourier modes."""
+    z, y, x = np.meshgrid(np.arange(prm.nz), np.arange(prm.ny), np.arange(prm.nx), indexing="ij")
+    out = []
+    for _ in range(3):
+        u = np.zeros((prm.nz, prm.ny, prm.nx))
+        for _m in range(4):
+            k = rng.integers(-kmax, kmax + 1, size=3)
+            ph = rng.uniform(0, 2 * np.pi)
+            u += np.cos(2 * np.pi * (k[0] * x / prm.nx + k[1] * y / prm.ny + k[2] * z / prm.nz) + ph)
+        out.append(amp * u / 4.0)
+    if prm.walls_y:
+        # vanish towards the walls
+        prof = np.sin(np.pi * (y + 0.5) / prm.ny)
+        out = [u * prof for u in out]
+    return out
+
+
+def start_both(prm, lat, orc, rng, amp=0.02):
+    """initpop from a smooth velocity field, then rhoupdat, on both sides
+    (BOX/main.f90:73-81)."""
+    ux, uy, uz = smooth_velocity(prm, rng, amp)
+    orc.ux[:], orc.uy[:], orc.uz[:] = ux, uy, uz
+    orc.initpop()
+    orc.rhoupdat()
+    lat.upload_macro(None, ux, uy, uz)
+    lat.initpop()
+    lat.rhoupdat()
+
+
+def fluid_step_both(lat, orc):
+    """FORCING, collision_MRT, streaming, macrovar on both sides."""
+    if orc.P.forcing_on:
+        orc.forcing()
+        lat.forcing()
+    orc.collision_mrt()
+    lat.collision_mrt()
+    orc.streaming()
+    lat.streaming()
+    orc.macrovar()
+    lat.macrovar()
+
+
+def place_spheres(prm, rng, n, gap=3.0, margin=0.0):
+    """n non-overlapping random centres (surface gap >= gap), avoiding the y
+    walls by `margin` when walls are on."""
+    pos = []
+    tries = 0
+    while len(pos) < n:
+        tries += 1
+        assert tries < 100000
+        c = rng.uniform([0, margin, 0], [prm.nx, prm.ny - margin, prm.nz])
+        ok = True
+        for q in pos:
+            d = c - q
+            for k, L in enumerate((prm.nx, prm.ny, prm.nz)):
+                if k == 1 and prm.walls_y:
+                    continue
+                d[k] -= L * np.round(d[k] / L)
+            if np.linalg.norm(d) < 2 * prm.rad + gap:
+                ok = False
+                break
+        if ok:
+            pos.append(c)
+    return np.array(pos)
