@@ -7,13 +7,16 @@ import numpy as np
 from multiphase_flow_codes_amd.params import LBMParams
 
 
-def small_params(n=(20, 24, 16), npart=0, rad=3.0, forcing=True, walls=False, visc=0.01, **kw):
+def small_params(n=(20, 24, 16), npart=0, rad=3.0, forcing=True, walls=False, visc=0.01,
+                 strong_forcing=True, **kw):
     nx, ny, nz = n
     prm = LBMParams(nx=nx, ny=ny, nz=nz, visc=visc, npart=npart, rad=rad, forcing=forcing,
                     walls_y=walls, **kw)
-    # strong forcing so the forced modes matter within a few steps
-    prm.vscale = 0.05
-    prm.derive()
+    if strong_forcing:
+        # forcing strong enough to matter within a few steps (parity tests
+        # only: over hundreds of steps it drives the flow unstable)
+        prm.vscale = 0.01
+        prm.derive()
     return prm
 
 
@@ -60,10 +63,10 @@ def fluid_step_both(lat, orc):
     lat.macrovar()
 
 
-def place_spheres(prm, rng, n, gap=3.0, margin=0.0):
+def place_spheres(prm, rng, n, gap=3.0, margin=0.0, first=None):
     """n non-overlapping random centres (surface gap >= gap), avoiding the y
-    walls by `margin` when walls are on."""
-    pos = []
+    walls by `margin` when walls are on; `first` fixes the first centre."""
+    pos = [] if first is None else [np.asarray(first, dtype=float)]
     tries = 0
     while len(pos) < n:
         tries += 1
