@@ -55,7 +55,7 @@ def run_gpu(case, precision, steps, warmup, field):
     avg_kern = kern_ms / max(launches, 1)
     achieved = BYTES_PER_CELL[precision] * ncell / (avg_kern * 1e-3) / 1e9
     log(f"{precision}: {ms_step:.3f} ms/step, {mlups:.0f} MLUPS, collide {avg_kern:.3f} ms "
-        f"({achieved:.0f} GB/s), links {st['nlinks']}, refills/step {st['refill']}")
+        f"({achieved:.0f} GB/s), links {st['nlinks']}, refilled nodes {st['refill']}")
     lat.close()
     return {
         "mlups": mlups, "ms_per_step": ms_step, "kernel_ms": avg_kern, "launches": launches,

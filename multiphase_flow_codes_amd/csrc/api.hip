@@ -257,6 +257,18 @@ int check_faults(mfc_ctx* ctx) {
   return MFC_OK;
 }
 
+// beads_links into the map buffer that does not hold the previous step's map
+// (the one beads_filling still needs), clearing what that buffer marked
+// before.  With `overlap_check` the links pass also runs ovlpchk.
+int links_now(mfc_ctx* ctx, int overlap_check, int istep) {
+  Ctx& c = ctx->c;
+  Particles& P = c.pt;
+  const int target = (P.map_cur != P.map_old) ? P.map_cur : (P.map_cur ^ 1);
+  HIPCHK(launch_links(c, target, overlap_check, istep));
+  P.map_cur = target;
+  return MFC_OK;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -348,10 +360,11 @@ int mfc_destroy(mfc_ctx* ctx) {
   Particles& P = c.pt;
   double* pd[] = {P.yp, P.ypglb, P.ypglbp, P.wp, P.wpp, P.omgp, P.omgpp, P.fHIp, P.flubp, P.torqp,
                   P.forcep, P.forcepp, P.torqpp, P.dwdt, P.domgdt, P.thetap, P.map_pos[0], P.map_pos[1],
-                  P.lk_alpha, P.lk_out, P.scratch};
+                  P.lk_alpha, P.lk_out, P.seg_ft};
   for (double* a : pd) F(a);
   F(P.isn[0]); F(P.isn[1]); F(P.mask[0]); F(P.mask[1]);
   F(P.lk_node); F(P.lk_dir); F(P.lk_cnt); F(P.lk_off); F(P.sol_cnt);
+  F(P.fill_list); F(P.fill_count);
   F(c.d_status); F(c.d_red); F(c.staging);
   if (c.h_modes) hipHostFree(c.h_modes);
   if (c.h_status) hipHostFree(c.h_status);
@@ -622,12 +635,23 @@ int mfc_prt_init(mfc_ctx* ctx, int32_t npart) {
       *a = nullptr;
       HIPCHK(hipMalloc(a, sizeof(double) * 3 * npart));
     }
+    P.nseg = segments_per_particle(c.p.rad);
+    const size_t nsegs = (size_t)npart * P.nseg;
     if (P.lk_cnt) HIPCHK(hipFree(P.lk_cnt));
     if (P.sol_cnt) HIPCHK(hipFree(P.sol_cnt));
     if (P.lk_off) HIPCHK(hipFree(P.lk_off));
-    HIPCHK(hipMalloc(&P.lk_cnt, sizeof(int32_t) * npart));
-    HIPCHK(hipMalloc(&P.sol_cnt, sizeof(int32_t) * npart));
-    HIPCHK(hipMalloc(&P.lk_off, sizeof(int64_t) * (npart + 1)));
+    if (P.seg_ft) HIPCHK(hipFree(P.seg_ft));
+    HIPCHK(hipMalloc(&P.lk_cnt, sizeof(int32_t) * nsegs));
+    HIPCHK(hipMalloc(&P.sol_cnt, sizeof(int32_t) * nsegs));
+    HIPCHK(hipMalloc(&P.lk_off, sizeof(int64_t) * (nsegs + 1)));
+    HIPCHK(hipMalloc(&P.seg_ft, sizeof(double) * 6 * nsegs));
+    // every refilled node was solid: at most the solid nodes of all spheres
+    const double rs = c.p.rad + 1.0;
+    P.fill_cap = (int64_t)ceil(4.0 / 3.0 * M_PI * rs * rs * rs) * npart + 64;
+    if (P.fill_list) HIPCHK(hipFree(P.fill_list));
+    if (P.fill_count) HIPCHK(hipFree(P.fill_count));
+    HIPCHK(hipMalloc(&P.fill_list, sizeof(uint64_t) * P.fill_cap));
+    HIPCHK(hipMalloc(&P.fill_count, sizeof(int32_t)));
     // link pool: every link's node lies in the shell r < d <= r + sqrt(2)
     // around its centre, and a node outside a convex sphere sees it along at
     // most 9 of the 18 directions (never along both c and -c).
@@ -668,7 +692,8 @@ int mfc_prt_init(mfc_ctx* ctx, int32_t npart) {
     double* pd[] = {P.yp, P.ypglb, P.ypglbp, P.wp, P.wpp, P.omgp, P.omgpp, P.fHIp, P.flubp, P.torqp,
                     P.forcep, P.forcepp, P.torqpp, P.dwdt, P.domgdt, P.thetap};
     for (double* a : pd) HIPCHK(hipMemsetAsync(a, 0, sizeof(double) * 3 * npart, c.stream));
-    HIPCHK(hipMemsetAsync(P.lk_off, 0, sizeof(int64_t) * (npart + 1), c.stream));
+    HIPCHK(hipMemsetAsync(P.lk_off, 0, sizeof(int64_t) * ((size_t)npart * P.nseg + 1), c.stream));
+    HIPCHK(hipMemsetAsync(P.fill_count, 0, sizeof(int32_t), c.stream));
   }
   HIPCHK(hipMemsetAsync(c.d_status, 0, sizeof(int32_t) * ST_COUNT, c.stream));
   HIPCHK(hipStreamSynchronize(c.stream));
@@ -787,12 +812,7 @@ int mfc_beads_links(mfc_ctx* ctx) {
   Ctx& c = ctx->c;
   Particles& P = c.pt;
   if (P.np <= 0) return MFC_ERR_ARG;
-  const int target = (P.map_cur != P.map_old) ? P.map_cur : (P.map_cur ^ 1);
-  HIPCHK(set_status(c, ST_ALPHA_FAULT, 0));
-  HIPCHK(launch_clear_map(c, target));
-  HIPCHK(launch_links(c, target));
-  P.map_cur = target;
-  return MFC_OK;
+  return links_now(ctx, 0, 0);
 }
 
 int mfc_beads_collision(mfc_ctx* ctx) {
@@ -800,7 +820,6 @@ int mfc_beads_collision(mfc_ctx* ctx) {
   Ctx& c = ctx->c;
   if (c.pt.np <= 0 || c.pt.map_np[c.pt.map_cur] == 0) return fail(ctx, MFC_ERR_STATE, "beads_collision before beads_links");
   if (c.phase != PHASE_PRE_STREAM) return fail(ctx, MFC_ERR_STATE, "beads_collision expects the post-collision field");
-  HIPCHK(set_status(c, ST_CONTACT_LINKS, 0));
   HIPCHK(launch_bounce_back(c));
   c.f_version++;
   return MFC_OK;
@@ -817,7 +836,7 @@ int mfc_beads_move(mfc_ctx* ctx, int32_t istep) {
   NEEDCTX;
   Ctx& c = ctx->c;
   if (c.pt.np <= 0) return MFC_ERR_ARG;
-  HIPCHK(launch_move(c, istep == c.p.irelease ? 1 : 0));
+  HIPCHK(launch_move(c, istep == c.p.irelease ? 1 : 0, 0));
   c.pt.map_old = c.pt.map_cur;   // ibnodes0 = ibnodes (partlib.f90:2376-2377)
   return MFC_OK;
 }
@@ -835,7 +854,6 @@ int mfc_beads_filling(mfc_ctx* ctx) {
   Ctx& c = ctx->c;
   if (c.pt.np <= 0) return MFC_ERR_ARG;
   if (c.phase != PHASE_POST_STREAM) return fail(ctx, MFC_ERR_STATE, "beads_filling expects the streamed field");
-  HIPCHK(set_status(c, ST_REFILL, 0));
   HIPCHK(launch_filling(c));
   c.f_version++;
   return MFC_OK;
@@ -894,10 +912,12 @@ int mfc_step(mfc_ctx* ctx, int32_t istep, int32_t nsteps) {
     if (on && (rc = mfc_beads_collision(ctx))) return rc;
     if ((rc = mfc_streaming(ctx))) return rc;
     if (on) {
+      // lubforce; move with the periodic wrap of beads_redistribute fused in;
+      // links with its overlap check fused in (reported by mfc_sync)
       if ((rc = mfc_beads_lubforce(ctx))) return rc;
-      if ((rc = mfc_beads_move(ctx, it))) return rc;
-      HIPCHK(launch_redistribute(c, it));   // overlap reported by the next mfc_sync
-      if ((rc = mfc_beads_links(ctx))) return rc;
+      HIPCHK(launch_move(c, it == c.p.irelease ? 1 : 0, 1));
+      c.pt.map_old = c.pt.map_cur;
+      if ((rc = links_now(ctx, 1, it))) return rc;
       if ((rc = mfc_beads_filling(ctx))) return rc;
     }
     if ((rc = mfc_macrovar(ctx))) return rc;

@@ -45,26 +45,67 @@ __device__ __forceinline__ void force_at(const double* __restrict__ G, const dou
   Fz = v[2];
 }
 
-// Fused stream + macro + forcing + MRT collision.  ARR: conserved moments
-// from the rho/u arrays (the reference's literal data flow, used while they
-// are authoritative, e.g. the pre-relaxation); otherwise recomputed from the
-// pulled populations plus half the forcing captured at the last macrovar,
-// which is exactly what those arrays would hold.
+// Forcing field at node x from one row of a forcing table, in precision R.
+template <typename R>
+__device__ __forceinline__ void force_row(const double* __restrict__ t, R c1, R s1, R c2, R s2,
+                                          R& Fx, R& Fy, R& Fz) {
+  R v[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double* q = t + 6 * c;
+    v[c] = (R)q[0] + (R)2 * ((R)q[2] * c1 - (R)q[3] * s1) + (R)2 * ((R)q[4] * c2 - (R)q[5] * s2);
+  }
+  Fx = v[0];
+  Fy = v[1];
+  Fz = v[2];
+}
+
+// Fused stream + macro + forcing + MRT collision.
+//
+// Launch: blockIdx.z = z, blockIdx.y = y, blockIdx.x * blockDim.x + lane = x.
+// Everything that depends on (y, z) only -- the nine neighbour-row offsets,
+// the wall test and the forcing-table rows -- is wave-uniform and lives in
+// scalar registers; per lane only x +- 1 wraps.  Stores are non-temporal:
+// the post-collision field is not re-read until the next step.
+//
+// ARR: conserved moments from the rho/u arrays (the reference's literal data
+// flow, used while they are authoritative, e.g. the pre-relaxation);
+// otherwise recomputed from the pulled populations plus half the forcing
+// captured at the last macrovar, which is exactly what those arrays hold.
 template <typename T, typename R, bool ARR, bool WALLS>
 __global__ void __launch_bounds__(256)
     k_collide(const T* __restrict__ A, T* __restrict__ B, const uint32_t* __restrict__ mask, Grid g,
               CollideConsts k, ForceArgs fa, const double* __restrict__ rho_a,
               const double* __restrict__ ux_a, const double* __restrict__ uy_a,
               const double* __restrict__ uz_a) {
-  const uint32_t n = blockIdx.x * 256u + threadIdx.x;
-  if (n >= g.ncell) return;
-  int x, y, z;
-  uint32_t row;
-  node_coords(n, g, x, y, z, row);
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y, z = blockIdx.z;
+  if (x >= g.nx) return;
+  const uint32_t nx = (uint32_t)g.nx, ny = (uint32_t)g.ny;
+  const uint32_t ym = y == 0 ? ny - 1 : y - 1, yp = y == g.ny - 1 ? 0 : y + 1;
+  const uint32_t zm = z == 0 ? g.nz - 1 : z - 1, zp = z == g.nz - 1 ? 0 : z + 1;
+  const uint32_t row = (uint32_t)y + ny * (uint32_t)z;
+  // neighbour rows, indexed [dy+1][dz+1] for the source row (y - cy, z - cz)
+  const uint32_t yy[3] = {yp, (uint32_t)y, ym};   // cy = -1, 0, +1
+  const uint32_t zz[3] = {zp, (uint32_t)z, zm};   // cz = -1, 0, +1
+  const uint32_t xm = x == 0 ? nx - 1 : x - 1, xpp = x == g.nx - 1 ? 0 : x + 1;
+  const uint32_t xs[3] = {xpp, (uint32_t)x, xm};   // cx = -1, 0, +1
+  const uint32_t n = (uint32_t)x + nx * row;
+  const bool wall_lo = WALLS && y == 0, wall_hi = WALLS && y == g.ny - 1;
   R f[NPOP];
-  load_pulled<T, R, WALLS>(A, g, x, y, z, n, f);
+#pragma unroll
+  for (int i = 0; i < NPOP; ++i) {
+    const int ccx = cx(i), ccy = cy(i), ccz = cz(i);
+    const bool wall = (ccy > 0 && wall_lo) || (ccy < 0 && wall_hi);
+    const uint32_t src = wall ? n : xs[ccx + 1] + nx * (yy[ccy + 1] + ny * zz[ccz + 1]);
+    const int pl = wall ? opp(i) : i;
+    f[i] = (R)A[(uint64_t)pl * g.ncell + src];
+  }
   const bool solid = mask != nullptr && ((mask[n >> 5] >> (n & 31u)) & 1u);
   if (!solid) {
+    const int x2 = (2 * x >= g.nx) ? 2 * x - g.nx : 2 * x;
+    const R c1 = (R)fa.phx[2 * x], s1 = (R)fa.phx[2 * x + 1];
+    const R c2 = (R)fa.phx[2 * x2], s2 = (R)fa.phx[2 * x2 + 1];
     R rho, ux, uy, uz;
     if (ARR) {
       rho = (R)rho_a[n];
@@ -74,30 +115,30 @@ __global__ void __launch_bounds__(256)
     } else {
       R jx, jy, jz;
       density_momentum<R>(f, rho, jx, jy, jz);
-      double mx = fa.bfx, my = fa.bfy, mz = fa.bfz;
+      R mx = (R)fa.bfx, my = (R)fa.bfy, mz = (R)fa.bfz;
       if (fa.gmac) {
-        double gx, gy, gz;
-        force_at(fa.gmac, fa.phx, row, x, g.nx, gx, gy, gz);
+        R gx, gy, gz;
+        force_row<R>(fa.gmac + 18ull * row, c1, s1, c2, s2, gx, gy, gz);
         mx += gx;
         my += gy;
         mz += gz;
       }
-      ux = jx + (R)(0.5 * mx);
-      uy = jy + (R)(0.5 * my);
-      uz = jz + (R)(0.5 * mz);
+      ux = jx + (R)0.5 * mx;
+      uy = jy + (R)0.5 * my;
+      uz = jz + (R)0.5 * mz;
     }
-    double Fx = fa.bfx, Fy = fa.bfy, Fz = fa.bfz;
+    R Fx = (R)fa.bfx, Fy = (R)fa.bfy, Fz = (R)fa.bfz;
     if (fa.gcur) {
-      double gx, gy, gz;
-      force_at(fa.gcur, fa.phx, row, x, g.nx, gx, gy, gz);
+      R gx, gy, gz;
+      force_row<R>(fa.gcur + 18ull * row, c1, s1, c2, s2, gx, gy, gz);
       Fx += gx;
       Fy += gy;
       Fz += gz;
     }
-    mrt_collide<R>(f, rho, ux, uy, uz, (R)Fx, (R)Fy, (R)Fz, k);
+    mrt_collide<R>(f, rho, ux, uy, uz, Fx, Fy, Fz, k);
   }
 #pragma unroll
-  for (int i = 0; i < NPOP; ++i) B[(uint64_t)i * g.ncell + n] = (T)f[i];
+  for (int i = 0; i < NPOP; ++i) __builtin_nontemporal_store((T)f[i], &B[(uint64_t)i * g.ncell + n]);
 }
 
 // Forcing table of one step: per (y,z) row the nine partial sums
@@ -316,11 +357,20 @@ hipError_t launch_force_table(Ctx& c, int slot) {
   return hipGetLastError();
 }
 
+// x-row blocks: 64..256 lanes covering a row, one block per (x-chunk, y, z)
+static void row_launch(const Grid& g, dim3& grid, dim3& blk) {
+  int bx = ((g.nx + 63) / 64) * 64;
+  if (bx > 256) bx = 256;
+  blk = dim3(bx);
+  grid = dim3((g.nx + bx - 1) / bx, g.ny, g.nz);
+}
+
 template <typename T, typename R>
 static void collide_dispatch(Ctx& c, bool arrays, const uint32_t* mask, ForceArgs fa) {
   const T* A = (const T*)c.pop[c.cur];
   T* B = (T*)c.pop[c.cur ^ 1];
-  const dim3 grid(nblocks(c.g.ncell)), blk(256);
+  dim3 grid, blk;
+  row_launch(c.g, grid, blk);
   if (arrays) {
     if (c.g.walls_y)
       hipLaunchKernelGGL((k_collide<T, R, true, true>), grid, blk, 0, c.stream, A, B, mask, c.g, c.cc,

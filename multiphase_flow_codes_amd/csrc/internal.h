@@ -14,15 +14,17 @@ namespace mfc {
 enum Phase { PHASE_PRE_STREAM = 0, PHASE_POST_STREAM = 1 };
 enum MomState { MOM_NONE = 0, MOM_ARRAYS = 1, MOM_VIRTUAL = 2 };
 
-// device status words (int32), read back by mfc_sync / mfc_status
+// device status words (int32), read back by mfc_sync / mfc_status.  Counters
+// accumulate from mfc_prt_init on; ST_NLINKS / ST_NSOLID describe the last
+// beads_links.
 enum StatusWord {
-  ST_OVERLAP = 0,        // particles flagged by the last overlap check
+  ST_OVERLAP = 0,        // overlap flags raised (fatal in the reference)
   ST_OVERLAP_STEP = 1,   // first step number at which an overlap was seen (0: none)
-  ST_LINK_OVERFLOW = 2,  // links that did not fit the link pool
+  ST_LINK_OVERFLOW = 2,  // link builds that did not fit the link pool
   ST_ALPHA_FAULT = 3,    // wall fractions outside [0,1] (reported, not fatal)
   ST_CONTACT_LINKS = 4,  // boundary links whose own node is solid (near contact)
   ST_NONFINITE = 5,      // particles with non-finite or out-of-box state
-  ST_REFILL = 6,         // nodes refilled by the last beads_filling
+  ST_REFILL = 6,         // nodes refilled
   ST_NLINKS = 7,         // links built by the last beads_links
   ST_NSOLID = 8,         // solid nodes marked by the last beads_links
   ST_BOX_FAULT = 9,      // particle boxes that did not fit the lattice
@@ -60,10 +62,15 @@ struct Particles {
   uint8_t* lk_dir = nullptr;
   double* lk_alpha = nullptr;
   double* lk_out = nullptr;
-  int32_t* lk_cnt = nullptr;   // per particle
-  int64_t* lk_off = nullptr;   // per particle (np + 1)
-  int32_t* sol_cnt = nullptr;  // solid nodes per particle
-  double* scratch = nullptr;   // reduction scratch
+  int nseg = 1;                // box segments (z-slabs) per particle
+  int32_t* lk_cnt = nullptr;   // links per segment
+  int64_t* lk_off = nullptr;   // link offset per segment (np * nseg + 1)
+  int32_t* sol_cnt = nullptr;  // solid nodes per segment
+  double* seg_ft = nullptr;    // force and torque per segment (6 each)
+  // nodes to refill, found by beads_filling's first pass
+  uint64_t* fill_list = nullptr;
+  int32_t* fill_count = nullptr;
+  int64_t fill_cap = 0;
 };
 
 struct Ctx {
@@ -115,11 +122,11 @@ hipError_t launch_aos_to_pop(Ctx& c, const double* src, int z0, int nzs, bool pu
 hipError_t launch_force_field(Ctx& c, int slot, double* fx, double* fy, double* fz);
 
 // ---- launchers (particles.hip) ----
-hipError_t launch_links(Ctx& c, int target);
-hipError_t launch_clear_map(Ctx& c, int target);
+int segments_per_particle(double rad);
+hipError_t launch_links(Ctx& c, int target, int overlap_check, int istep);
 hipError_t launch_bounce_back(Ctx& c);
 hipError_t launch_lubforce(Ctx& c);
-hipError_t launch_move(Ctx& c, int first);
+hipError_t launch_move(Ctx& c, int first, int wrap);
 hipError_t launch_redistribute(Ctx& c, int istep);
 hipError_t launch_filling(Ctx& c);
 hipError_t launch_initpartvel(Ctx& c);

@@ -21,10 +21,20 @@
 namespace mfc {
 
 constexpr int PB = 256;  // threads per particle workgroup
+// A segment is one z-slab (SLAB planes) of one particle's bounding box: the
+// unit of work of every box-scan kernel, so a sphere spreads over several
+// workgroups.  Segment s of particle p is block p * nseg + s.
+constexpr int SLAB = 4;
 
 struct PGeom {
   int nx, ny, nz, walls_y;
   double rad, radp2, r2;
+  // (r + 1.5)^2: a node farther than r + 1.5 from the centre is neither
+  // solid nor has a link, since every lattice vector is shorter than 1.5
+  // (the margin over sqrt(2) dwarfs any rounding of the squared distances)
+  double rlink2;
+  // (r + 0.5)^2: no solid node lies farther out than this
+  double rsolid2;
 };
 
 struct AxisRange {
@@ -60,49 +70,51 @@ __device__ double y_image(double ybase, double ypnt, const PGeom& G) {
   return G.walls_y ? ybase : near_image(ybase, ypnt, G.ny);
 }
 
+// one-period wrap (the box candidate ranges never span more than a period)
+__device__ __forceinline__ int wrap1(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+// Per-particle constants of a box scan (shared memory), computed by the whole
+// block: thread 0 sets the centre and candidate ranges, then every thread
+// tests one z or y candidate and the first passing index wins by atomicMin.
 __device__ void box_info(const double* __restrict__ pos, int p, const PGeom& G, BoxInfo& B) {
-  B.x0 = pos[3 * p];
-  B.y0 = pos[3 * p + 1];
-  B.z0 = pos[3 * p + 2];
-  B.ok = sane_coord(B.x0, G.nx) && sane_coord(B.y0, G.ny) && sane_coord(B.z0, G.nz);
-  if (!B.ok) return;
-  B.xr = sweep_rest_base(B.x0, G.nx);
-  B.yr = G.walls_y ? B.y0 : sweep_rest_base(B.y0, G.ny);
-  B.ax = axis_range(B.x0, G.radp2);
-  B.ay = axis_range(B.y0, G.radp2);
-  B.az = axis_range(B.z0, G.radp2);
-  if (B.ax.cnt > G.nx || B.ay.cnt > G.ny || B.az.cnt > G.nz) {
-    B.ok = 0;
-    return;
+  if (threadIdx.x == 0) {
+    B.x0 = pos[3 * p];
+    B.y0 = pos[3 * p + 1];
+    B.z0 = pos[3 * p + 2];
+    B.ok = sane_coord(B.x0, G.nx) && sane_coord(B.y0, G.ny) && sane_coord(B.z0, G.nz);
+    if (B.ok) {
+      B.xr = sweep_rest_base(B.x0, G.nx);
+      B.yr = G.walls_y ? B.y0 : sweep_rest_base(B.y0, G.ny);
+      B.ax = axis_range(B.x0, G.radp2);
+      B.ay = axis_range(B.y0, G.radp2);
+      B.az = axis_range(B.z0, G.radp2);
+      if (B.ax.cnt > G.nx || B.ay.cnt > G.ny || B.az.cnt > G.nz) B.ok = 0;
+    }
+    B.kfirst = 1 << 30;
+    B.jfirst = 1 << 30;
   }
-  B.kfirst = 1 << 30;
-  for (int t = 0; t < B.az.cnt; ++t) {
-    const int kw = wrap_any(B.az.lo + t, G.nz);
-    const double zp = (double)(kw + 1) - 0.5;
-    const double zz0 = zp - near_image(B.z0, zp, G.nz);
-    if (fabs(zz0) <= G.radp2 && kw < B.kfirst) B.kfirst = kw;
+  __syncthreads();
+  if (B.ok) {
+    for (int t = threadIdx.x; t < B.az.cnt; t += blockDim.x) {
+      const int kw = wrap1(B.az.lo + t, G.nz);
+      const double zp = (double)(kw + 1) - 0.5;
+      const double zz0 = zp - near_image(B.z0, zp, G.nz);
+      if (fabs(zz0) <= G.radp2) atomicMin(&B.kfirst, kw);
+    }
+    for (int t = threadIdx.x; t < B.ay.cnt; t += blockDim.x) {
+      const int jw = G.walls_y ? B.ay.lo + t : wrap1(B.ay.lo + t, G.ny);
+      if (jw < 0 || jw >= G.ny) continue;
+      const double yp = (double)(jw + 1) - 0.5;
+      const double yy0 = yp - y_image(B.y0, yp, G);
+      if (fabs(yy0) <= G.radp2) atomicMin(&B.jfirst, jw);
+    }
   }
-  B.jfirst = 1 << 30;
-  for (int t = 0; t < B.ay.cnt; ++t) {
-    const int jw = G.walls_y ? B.ay.lo + t : wrap_any(B.ay.lo + t, G.ny);
-    if (jw < 0 || jw >= G.ny) continue;
-    const double yp = (double)(jw + 1) - 0.5;
-    const double yy0 = yp - y_image(B.y0, yp, G);
-    if (fabs(yy0) <= G.radp2 && jw < B.jfirst) B.jfirst = jw;
-  }
+  __syncthreads();
 }
 
-// Geometry of box node t.  Returns false if the node is outside the
-// reference's bounding box (or outside the walls).
-__device__ bool box_node(const BoxInfo& B, const PGeom& G, int t, int& iw, int& jw, int& kw,
-                         LinkGeom& L) {
-  const int tx = t % B.ax.cnt;
-  const int rest = t / B.ax.cnt;
-  const int ty = rest % B.ay.cnt;
-  const int tz = rest / B.ay.cnt;
-  kw = wrap_any(B.az.lo + tz, G.nz);
-  jw = G.walls_y ? B.ay.lo + ty : wrap_any(B.ay.lo + ty, G.ny);
-  iw = wrap_any(B.ax.lo + tx, G.nx);
+// Geometry of node (iw, jw, kw) of the box.  Returns false if the node is
+// outside the reference's bounding box (partlib.f90:965-985) or beyond a wall.
+__device__ bool node_geom(const BoxInfo& B, const PGeom& G, int iw, int jw, int kw, LinkGeom& L) {
   if (jw < 0 || jw >= G.ny) return false;
   const double zp = (double)(kw + 1) - 0.5;
   L.zimg = near_image(B.z0, zp, G.nz);
@@ -120,6 +132,31 @@ __device__ bool box_node(const BoxInfo& B, const PGeom& G, int t, int& iw, int& 
   return true;
 }
 
+// Walk of one segment (z-slab s) of the box in passes of PB candidate nodes:
+// each 32-lane group takes a row (fixed y, z), lanes take consecutive x.
+// Every thread runs the same number of passes (barriers are allowed in fn);
+// fn(active, iw, jw, kw) is called once per pass.
+template <typename F>
+__device__ __forceinline__ void walk_segment(const BoxInfo& B, const PGeom& G, int s, F&& fn) {
+  const int lane = threadIdx.x & 31, grp = threadIdx.x >> 5;
+  const int z0 = s * SLAB, z1 = min(B.az.cnt, z0 + SLAB);
+  const int nrows = z1 > z0 ? (z1 - z0) * B.ay.cnt : 0;
+  for (int r0 = 0; r0 < nrows; r0 += PB / 32) {
+    const int r = r0 + grp;
+    const bool rowok = r < nrows;
+    const int tz = rowok ? z0 + r / B.ay.cnt : 0;
+    const int ty = rowok ? r - (tz - z0) * B.ay.cnt : 0;
+    const int kw = wrap1(B.az.lo + tz, G.nz);
+    const int jw = G.walls_y ? B.ay.lo + ty : wrap1(B.ay.lo + ty, G.ny);
+    for (int tx0 = 0; tx0 < B.ax.cnt; tx0 += 32) {
+      const int tx = tx0 + lane;
+      const bool act = rowok && tx < B.ax.cnt;
+      const int iw = wrap1(B.ax.lo + (act ? tx : 0), G.nx);
+      fn(act, iw, jw, kw);
+    }
+  }
+}
+
 // Neighbour test of direction q from node (iw,jw,kw); fills rr1 and offsets.
 __device__ __forceinline__ bool link_hit(const LinkGeom& L, int iw, int jw, int kw, int q, double r2,
                                          double& xx1, double& yy1, double& zz1, double& rr1) {
@@ -130,214 +167,286 @@ __device__ __forceinline__ bool link_hit(const LinkGeom& L, int iw, int jw, int 
   return (rr1 - r2) <= 0.0;
 }
 
+// Exclusive prefix sum over the PB threads of a block: a shuffle scan inside
+// each 64-lane wavefront, then the wavefront totals through LDS.
 __device__ __forceinline__ int block_exclusive_scan(int v, int* sh, int& total) {
-  // Hillis-Steele inclusive scan over PB threads in shared memory
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int off = 1; off < PB; off <<= 1) {
-    const int a = (t >= off) ? sh[t - off] : 0;
-    __syncthreads();
-    sh[t] += a;
-    __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
   }
-  total = sh[PB - 1];
-  const int excl = sh[t] - v;
+  if (lane == 63) sh[w] = x;
   __syncthreads();
-  return excl;
+  int before = 0, tot = 0;
+#pragma unroll
+  for (int q = 0; q < PB / 64; ++q) {
+    const int t = sh[q];
+    before += (q < w) ? t : 0;
+    tot += t;
+  }
+  __syncthreads();
+  total = tot;
+  return before + x - v;
 }
 
-// Pass 1 of beads_links: count solid nodes and boundary links per particle.
+
+// Set (or clear) the solid bit of node n for every lane with `on`, issuing
+// one atomic per distinct mask word per wavefront: lanes walking a box row
+// share one or two words, and same-address atomics from one instruction
+// would otherwise serialise.  Must be called by every lane of the wavefront.
+__device__ __forceinline__ void mask_update(uint32_t* __restrict__ mask, bool on, uint32_t n, bool set) {
+  uint64_t pending = __ballot(on);
+  const uint32_t wd = n >> 5;
+  const uint32_t bit = on ? (1u << (n & 31u)) : 0u;
+  const int lane = threadIdx.x & 63;
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t wl = __shfl(wd, leader, 64);
+    const bool mem = on && wd == wl;
+    uint32_t v = mem ? bit : 0u;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+    if (lane == leader) {
+      if (set)
+        atomicOr(&mask[wl], v);
+      else
+        atomicAnd(&mask[wl], ~v);
+    }
+    pending &= ~__ballot(mem);
+  }
+}
+
+// Pass 1 of beads_links (partlib.f90:945-1043) for one segment of a
+// particle's box: count solid nodes and boundary links.  The same block first
+// clears the segment of the nodes this map buffer marked two steps earlier
+// (by id, so exactly those nodes are reset), and with `ovl` set the first
+// segment of each particle performs the overlap check of beads_redistribute
+// (partlib.f90:2631-2646) on the freshly wrapped positions.
 __global__ void __launch_bounds__(PB)
-    k_links_count(const double* __restrict__ ypglb, int np, PGeom G, int32_t* __restrict__ lk_cnt,
-                  int32_t* __restrict__ sol_cnt, int32_t* __restrict__ status) {
+    k_links_count(const double* __restrict__ ypglb, int np, int nseg, PGeom G,
+                  const double* __restrict__ clr_pos, int clr_np, int32_t* __restrict__ isn,
+                  uint32_t* __restrict__ mask, int32_t* __restrict__ lk_cnt,
+                  int32_t* __restrict__ sol_cnt, int ovl, int istep, int32_t* __restrict__ status) {
   __shared__ BoxInfo B;
-  __shared__ int sh[PB];
-  const int p = blockIdx.x;
+  __shared__ int sh[PB / 64];
+  __shared__ int hit;
+  const int b = blockIdx.x;
+  const int p = b / nseg, s = b - p * nseg;
+  if (p < clr_np) {
+    box_info(clr_pos, p, G, B);
+    if (B.ok)
+      walk_segment(B, G, s, [&](bool act, int iw, int jw, int kw) {
+        LinkGeom L;
+        const uint32_t n = (uint32_t)iw + (uint32_t)G.nx * ((uint32_t)jw + (uint32_t)G.ny * (uint32_t)kw);
+        const bool mine = act && node_geom(B, G, iw, jw, kw, L) && L.rr0 <= G.rsolid2 && isn[n] == p + 1;
+        if (mine) isn[n] = 0;
+        mask_update(mask, mine, n, false);
+      });
+    __syncthreads();
+  }
   if (p >= np) return;
-  if (threadIdx.x == 0) box_info(ypglb, p, G, B);
-  __syncthreads();
+  if (ovl && s == 0) {
+    if (threadIdx.x == 0) hit = 0;
+    __syncthreads();
+    // ovlpchk: flag p if any later particle is closer than 2r
+    const double xc = ypglb[3 * p], yc = ypglb[3 * p + 1], zc = ypglb[3 * p + 2];
+    for (int q = p + 1 + threadIdx.x; q < np; q += PB) {
+      const double xb = near_image(ypglb[3 * q], xc, G.nx);
+      const double yb = G.walls_y ? ypglb[3 * q + 1] : near_image(ypglb[3 * q + 1], yc, G.ny);
+      const double zb = near_image(ypglb[3 * q + 2], zc, G.nz);
+      const double dist = sqrt((xc - xb) * (xc - xb) + (yc - yb) * (yc - yb) + (zc - zb) * (zc - zb));
+      if (dist - 2.0 * G.rad < 0.0) hit = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && hit) {
+      atomicAdd(&status[ST_OVERLAP], 1);
+      atomicCAS(&status[ST_OVERLAP_STEP], 0, istep);
+    }
+  }
+  box_info(ypglb, p, G, B);
   if (!B.ok) {
     if (threadIdx.x == 0) {
-      lk_cnt[p] = 0;
-      sol_cnt[p] = 0;
-      atomicAdd(&status[ST_BOX_FAULT], 1);
+      lk_cnt[b] = 0;
+      sol_cnt[b] = 0;
+      if (s == 0) atomicAdd(&status[ST_BOX_FAULT], 1);
     }
     return;
   }
-  const int nbox = B.ax.cnt * B.ay.cnt * B.az.cnt;
   int nl = 0, ns = 0;
-  for (int t = threadIdx.x; t < nbox; t += PB) {
-    int iw, jw, kw;
+  walk_segment(B, G, s, [&](bool act, int iw, int jw, int kw) {
     LinkGeom L;
-    if (!box_node(B, G, t, iw, jw, kw, L)) continue;
+    if (!act || !node_geom(B, G, iw, jw, kw, L) || L.rr0 > G.rlink2) return;
     if (L.rr0 - G.r2 <= 0.0) {
       ++ns;
     } else {
       for (int q = 1; q < NPOP; ++q) {
-        double a, b, c, d;
-        if (link_hit(L, iw, jw, kw, q, G.r2, a, b, c, d)) ++nl;
+        double a1, a2, a3, a4;
+        if (link_hit(L, iw, jw, kw, q, G.r2, a1, a2, a3, a4)) ++nl;
       }
     }
-  }
-  int tot;
+  });
+  int tot, tots;
   block_exclusive_scan(nl, sh, tot);
-  int tots;
   block_exclusive_scan(ns, sh, tots);
   if (threadIdx.x == 0) {
-    lk_cnt[p] = tot;
-    sol_cnt[p] = tots;
+    lk_cnt[b] = tot;
+    sol_cnt[b] = tots;
   }
 }
 
-// Exclusive scan of the per-particle link counts (one workgroup).
+// Exclusive scan of the per-segment link counts (one workgroup of 1024):
+// each thread sums a contiguous run, the run totals are scanned in LDS.
 __global__ void __launch_bounds__(1024)
-    k_scan_counts(const int32_t* __restrict__ cnt, const int32_t* __restrict__ scnt, int np,
-                  int64_t* __restrict__ off, int64_t cap, int32_t* __restrict__ status) {
+    k_scan_counts(const int32_t* __restrict__ cnt, const int32_t* __restrict__ scnt, int nsegs,
+                  int64_t* __restrict__ off, int64_t cap, int32_t* __restrict__ status,
+                  int32_t* __restrict__ fill_count) {
+  if (threadIdx.x == 0) *fill_count = 0;   // the next beads_filling starts an empty list
   __shared__ int64_t part[1024];
   __shared__ int64_t spart[1024];
   const int t = threadIdx.x;
-  const int per = (np + 1023) / 1024;
-  const int a = t * per, b = min(np, a + per);
+  const int per = (nsegs + 1023) / 1024;
+  const int a = min(nsegs, t * per), e = min(nsegs, a + per);
   int64_t s = 0, ss = 0;
-  for (int i = a; i < b; ++i) {
+  for (int i = a; i < e; ++i) {
     s += cnt[i];
     ss += scnt[i];
   }
   part[t] = s;
   spart[t] = ss;
   __syncthreads();
-  if (t == 0) {
-    int64_t acc = 0, acs = 0;
-    for (int i = 0; i < 1024; ++i) {
-      const int64_t v = part[i];
-      part[i] = acc;
-      acc += v;
-      acs += spart[i];
-    }
-    off[np] = acc;
-    status[ST_NLINKS] = (int32_t)min(acc, (int64_t)0x7fffffff);
-    status[ST_NSOLID] = (int32_t)min(acs, (int64_t)0x7fffffff);
-    if (acc > cap) atomicAdd(&status[ST_LINK_OVERFLOW], 1);
+  for (int d = 1; d < 1024; d <<= 1) {
+    const int64_t v = (t >= d) ? part[t - d] : 0;
+    const int64_t w = (t >= d) ? spart[t - d] : 0;
+    __syncthreads();
+    part[t] += v;
+    spart[t] += w;
+    __syncthreads();
   }
-  __syncthreads();
-  int64_t acc = part[t];
-  for (int i = a; i < b; ++i) {
+  int64_t acc = part[t] - s;
+  for (int i = a; i < e; ++i) {
     off[i] = acc;
     acc += cnt[i];
+  }
+  if (t == 1023) {
+    off[nsegs] = part[1023];
+    status[ST_NLINKS] = (int32_t)min(part[1023], (int64_t)0x7fffffff);
+    status[ST_NSOLID] = (int32_t)min(spart[1023], (int64_t)0x7fffffff);
+    if (part[1023] > cap) atomicAdd(&status[ST_LINK_OVERFLOW], 1);
   }
 }
 
 // Pass 2 of beads_links: mark solid nodes (id map + mask) and write the links
-// of particle p, in box-scan order, at its offset.
+// of each segment at the segment's offset, in walk order.  The first segment
+// of a particle records the centre the map was built from.
 __global__ void __launch_bounds__(PB)
-    k_links_write(const double* __restrict__ ypglb, int np, PGeom G, const int64_t* __restrict__ off,
-                  int64_t cap, int32_t* __restrict__ isn, uint32_t* __restrict__ mask,
-                  uint32_t* __restrict__ lk_node, uint8_t* __restrict__ lk_dir,
-                  double* __restrict__ lk_alpha, int32_t* __restrict__ status) {
+    k_links_write(const double* __restrict__ ypglb, int np, int nseg, PGeom G,
+                  const int64_t* __restrict__ off, int64_t cap, int32_t* __restrict__ isn,
+                  uint32_t* __restrict__ mask, double* __restrict__ mpos,
+                  uint32_t* __restrict__ lk_node, uint8_t* __restrict__ lk_dir) {
   __shared__ BoxInfo B;
-  __shared__ int sh[PB];
-  const int p = blockIdx.x;
+  __shared__ int sh[PB / 64];
+  const int b = blockIdx.x;
+  const int p = b / nseg, s = b - p * nseg;
   if (p >= np) return;
-  if (threadIdx.x == 0) box_info(ypglb, p, G, B);
-  __syncthreads();
+  box_info(ypglb, p, G, B);
+  if (threadIdx.x == 0 && s == 0)
+    for (int c = 0; c < 3; ++c) mpos[3 * p + c] = ypglb[3 * p + c];
   if (!B.ok) return;
-  const int nbox = B.ax.cnt * B.ay.cnt * B.az.cnt;
-  int64_t base = off[p];
-  int alpha_bad = 0;
-  for (int t0 = 0; t0 < nbox; t0 += PB) {
-    const int t = t0 + threadIdx.x;
-    int iw = 0, jw = 0, kw = 0;
+  int64_t base = off[b];
+  walk_segment(B, G, s, [&](bool act, int iw, int jw, int kw) {
     LinkGeom L;
-    bool in = (t < nbox) && box_node(B, G, t, iw, jw, kw, L);
-    bool solid = in && (L.rr0 - G.r2 <= 0.0);
+    const bool in = act && node_geom(B, G, iw, jw, kw, L) && L.rr0 <= G.rlink2;
+    const bool solid = in && (L.rr0 - G.r2 <= 0.0);
     uint32_t hits = 0;
     if (in && !solid) {
       for (int q = 1; q < NPOP; ++q) {
-        double a, b, c, d;
-        if (link_hit(L, iw, jw, kw, q, G.r2, a, b, c, d)) hits |= 1u << q;
+        double a1, a2, a3, a4;
+        if (link_hit(L, iw, jw, kw, q, G.r2, a1, a2, a3, a4)) hits |= 1u << q;
       }
     }
     const uint32_t n = (uint32_t)iw + (uint32_t)G.nx * ((uint32_t)jw + (uint32_t)G.ny * (uint32_t)kw);
-    if (solid) {
-      isn[n] = p + 1;
-      atomicOr(&mask[n >> 5], 1u << (n & 31u));
-    }
+    if (solid) isn[n] = p + 1;
+    mask_update(mask, solid, n, true);
     int tot;
     const int excl = block_exclusive_scan(__popc(hits), sh, tot);
     int64_t w = base + excl;
-    for (int q = 1; q < NPOP; ++q) {
-      if (!(hits & (1u << q))) continue;
-      double xx1, yy1, zz1, rr1;
-      link_hit(L, iw, jw, kw, q, G.r2, xx1, yy1, zz1, rr1);
-      const double alpha = link_alpha(L, xx1, yy1, zz1, rr1, G.r2);
-      if (alpha < 0.0 || alpha > 1.0) ++alpha_bad;
+    while (hits) {
+      const int q = __ffs(hits) - 1;
+      hits &= hits - 1;
       if (w < cap) {
         lk_node[w] = n;
         lk_dir[w] = (uint8_t)q;
-        lk_alpha[w] = alpha;
       }
       ++w;
     }
     base += tot;
-  }
-  if (alpha_bad) atomicAdd(&status[ST_ALPHA_FAULT], alpha_bad);
+  });
 }
 
-// Clear the nodes a map buffer marked for particle p (by id, so exactly the
-// nodes that were set are reset).
+// Wall fraction of every link (partlib.f90:1013-1024), one link per lane:
+// the node geometry is rebuilt exactly as the links pass saw it (same box
+// constants, same centre bases), then alpha from the quadratic.
 __global__ void __launch_bounds__(PB)
-    k_clear_map(const double* __restrict__ pos, int np, PGeom G, int32_t* __restrict__ isn,
-                uint32_t* __restrict__ mask) {
+    k_link_alpha(const double* __restrict__ ypglb, int np, int nseg, PGeom G,
+                 const int64_t* __restrict__ off, int64_t cap, const uint32_t* __restrict__ lk_node,
+                 const uint8_t* __restrict__ lk_dir, double* __restrict__ lk_alpha,
+                 int32_t* __restrict__ status) {
   __shared__ BoxInfo B;
-  const int p = blockIdx.x;
+  const int b = blockIdx.x;
+  const int p = b / nseg;
   if (p >= np) return;
-  if (threadIdx.x == 0) box_info(pos, p, G, B);
-  __syncthreads();
+  box_info(ypglb, p, G, B);
   if (!B.ok) return;
-  const int nbox = B.ax.cnt * B.ay.cnt * B.az.cnt;
-  for (int t = threadIdx.x; t < nbox; t += PB) {
-    const int tx = t % B.ax.cnt, rest = t / B.ax.cnt, ty = rest % B.ay.cnt, tz = rest / B.ay.cnt;
-    const int kw = wrap_any(B.az.lo + tz, G.nz);
-    const int jw = G.walls_y ? B.ay.lo + ty : wrap_any(B.ay.lo + ty, G.ny);
-    const int iw = wrap_any(B.ax.lo + tx, G.nx);
-    if (jw < 0 || jw >= G.ny) continue;
-    const uint32_t n = (uint32_t)iw + (uint32_t)G.nx * ((uint32_t)jw + (uint32_t)G.ny * (uint32_t)kw);
-    if (isn[n] == p + 1) {
-      isn[n] = 0;
-      atomicAnd(&mask[n >> 5], ~(1u << (n & 31u)));
-    }
+  const int64_t a = off[b], e = min(off[b + 1], cap);
+  int alpha_bad = 0;
+  for (int64_t l = a + threadIdx.x; l < e; l += PB) {
+    const uint32_t n = lk_node[l];
+    const int q = lk_dir[l];
+    const int iw = (int)(n % (uint32_t)G.nx);
+    const uint32_t rw = n / (uint32_t)G.nx;
+    const int jw = (int)(rw % (uint32_t)G.ny), kw = (int)(rw / (uint32_t)G.ny);
+    LinkGeom L;
+    node_geom(B, G, iw, jw, kw, L);
+    double xx1, yy1, zz1, rr1;
+    link_hit(L, iw, jw, kw, q, G.r2, xx1, yy1, zz1, rr1);
+    const double alpha = link_alpha(L, xx1, yy1, zz1, rr1, G.r2);
+    if (alpha < 0.0 || alpha > 1.0) ++alpha_bad;
+    lk_alpha[l] = alpha;
   }
+  if (alpha_bad) atomicAdd(&status[ST_ALPHA_FAULT], alpha_bad);
 }
 
 __device__ __forceinline__ bool bit_at(const uint32_t* __restrict__ mask, uint32_t n) {
   return (mask[n >> 5] >> (n & 31u)) & 1u;
 }
 
-// beads_collision (partlib.f90:1363-1703) for the links of particle p:
-// interpolated bounce-back value of every link into lk_out, and the
-// momentum-exchange force and torque on the particle reduced in FP64 in a
-// fixed order.  All links read one snapshot of the post-collision field; the
+// beads_collision (partlib.f90:1363-1703), one segment of links per block:
+// the interpolated bounce-back value of every link into lk_out, and the
+// segment's momentum-exchange force and torque (FP64, fixed reduction order)
+// into seg_ft.  All links read one snapshot of the post-collision field; the
 // values are scattered into the solid nodes by k_bb_scatter afterwards.
 template <typename T>
 __global__ void __launch_bounds__(PB)
     k_bounce_back(const T* __restrict__ A, Grid g, PGeom G, const uint32_t* __restrict__ mask,
-                  const int64_t* __restrict__ off, int64_t cap, const uint32_t* __restrict__ lk_node,
-                  const uint8_t* __restrict__ lk_dir, const double* __restrict__ lk_alpha,
-                  double* __restrict__ lk_out, const double* __restrict__ ypglb,
-                  const double* __restrict__ wp, const double* __restrict__ omgp, double ww1,
-                  double ww2, double* __restrict__ fHIp, double* __restrict__ torqp,
-                  int32_t* __restrict__ status) {
-  __shared__ double red[6][PB];
-  const int p = blockIdx.x;
-  const int64_t a = off[p];
-  const int64_t b = min(off[p + 1], cap);
+                  int nseg, const int64_t* __restrict__ off, int64_t cap,
+                  const uint32_t* __restrict__ lk_node, const uint8_t* __restrict__ lk_dir,
+                  const double* __restrict__ lk_alpha, double* __restrict__ lk_out,
+                  const double* __restrict__ ypglb, const double* __restrict__ wp,
+                  const double* __restrict__ omgp, double ww1, double ww2,
+                  double* __restrict__ seg_ft, int32_t* __restrict__ status) {
+  __shared__ double red[6][PB / 64];
+  const int b = blockIdx.x;
+  const int p = b / nseg;
+  const int64_t a = off[b];
+  const int64_t e = min(off[b + 1], cap);
   const double xcp = ypglb[3 * p], ycp = ypglb[3 * p + 1], zcp = ypglb[3 * p + 2];
   const double w1 = wp[3 * p], w2 = wp[3 * p + 1], w3 = wp[3 * p + 2];
   const double o1 = omgp[3 * p], o2 = omgp[3 * p + 1], o3 = omgp[3 * p + 2];
   double acc[6] = {0, 0, 0, 0, 0, 0};
   int contact = 0;
-  for (int64_t l = a + threadIdx.x; l < b; l += PB) {
+  for (int64_t l = a + threadIdx.x; l < e; l += PB) {
     const uint32_t n = lk_node[l];
     const int q = lk_dir[l];
     const double alpha = lk_alpha[l];
@@ -357,12 +466,12 @@ __global__ void __launch_bounds__(PB)
     const double uwy = w2 + o3 * xx0 - o1 * zz0;
     const double uwz = w3 + o1 * yy0 - o2 * xx0;
     const double uwpro = uwx * (double)ix + uwy * (double)iy + uwz * (double)iz;
-    // x - c and x - 2c (periodic; outside the walls counts as solid)
+    // x - c and x - 2c (periodic; beyond a wall counts as solid)
     const int y1 = y - iy, y2 = y - 2 * iy;
     const bool out1 = g.walls_y && (y1 < 0 || y1 >= g.ny);
     const bool out2 = g.walls_y && (y2 < 0 || y2 >= g.ny);
-    const uint32_t m1 = out1 ? n : (uint32_t)wrap_any(x - ix, g.nx) + (uint32_t)g.nx * ((uint32_t)wrap_any(y1, g.ny) + (uint32_t)g.ny * (uint32_t)wrap_any(z - iz, g.nz));
-    const uint32_t m2 = out2 ? n : (uint32_t)wrap_any(x - 2 * ix, g.nx) + (uint32_t)g.nx * ((uint32_t)wrap_any(y2, g.ny) + (uint32_t)g.ny * (uint32_t)wrap_any(z - 2 * iz, g.nz));
+    const uint32_t m1 = out1 ? n : (uint32_t)wrap1(x - ix, g.nx) + (uint32_t)g.nx * ((uint32_t)wrap1(y1, g.ny) + (uint32_t)g.ny * (uint32_t)wrap1(z - iz, g.nz));
+    const uint32_t m2 = out2 ? n : (uint32_t)wrap1(x - 2 * ix, g.nx) + (uint32_t)g.nx * ((uint32_t)wrap1(y2, g.ny) + (uint32_t)g.ny * (uint32_t)wrap1(z - 2 * iz, g.nz));
     const bool sol1 = out1 || bit_at(mask, m1);
     const bool sol2 = out2 || bit_at(mask, m2);
     const int qo = opp(q);
@@ -384,38 +493,47 @@ __global__ void __launch_bounds__(PB)
     acc[4] += dxm * zz0 - dzm * xx0;
     acc[5] += dym * xx0 - dxm * yy0;
   }
+  // wavefront tree reduction (fixed order), then the four wave partials
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-  for (int c = 0; c < 6; ++c) red[c][threadIdx.x] = acc[c];
-  __syncthreads();
-  for (int s = PB / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
+  for (int c = 0; c < 6; ++c) {
+    double v = acc[c];
 #pragma unroll
-      for (int c = 0; c < 6; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + s];
-    }
-    __syncthreads();
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) red[c][w] = v;
   }
-  if (threadIdx.x == 0) {
-    fHIp[3 * p] = red[0][0];
-    fHIp[3 * p + 1] = red[1][0];
-    fHIp[3 * p + 2] = red[2][0];
-    torqp[3 * p] = red[3][0];
-    torqp[3 * p + 1] = red[4][0];
-    torqp[3 * p + 2] = red[5][0];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < PB / 64; ++q) v += red[threadIdx.x][q];
+    seg_ft[6ll * b + threadIdx.x] = v;
   }
   if (contact) atomicAdd(&status[ST_CONTACT_LINKS], contact);
 }
 
 // Write each link's bounce-back value into the opposite population of the
-// solid neighbour x + c (partlib.f90:1605-1621).
+// solid neighbour x + c (partlib.f90:1605-1621); the first segment of each
+// particle also sums its segments' force and torque into fHIp / torqp.
 template <typename T>
 __global__ void __launch_bounds__(PB)
-    k_bb_scatter(T* __restrict__ A, Grid g, const int64_t* __restrict__ off, int64_t cap,
+    k_bb_scatter(T* __restrict__ A, Grid g, int nseg, const int64_t* __restrict__ off, int64_t cap,
                  const uint32_t* __restrict__ lk_node, const uint8_t* __restrict__ lk_dir,
-                 const double* __restrict__ lk_out) {
-  const int p = blockIdx.x;
-  const int64_t a = off[p];
-  const int64_t b = min(off[p + 1], cap);
-  for (int64_t l = a + threadIdx.x; l < b; l += PB) {
+                 const double* __restrict__ lk_out, const double* __restrict__ seg_ft,
+                 double* __restrict__ fHIp, double* __restrict__ torqp) {
+  const int b = blockIdx.x;
+  const int p = b / nseg;
+  if (b - p * nseg == 0 && threadIdx.x < 6) {
+    double v = 0.0;
+    for (int s = 0; s < nseg; ++s) v += seg_ft[6ll * (b + s) + threadIdx.x];
+    if (threadIdx.x < 3)
+      fHIp[3 * p + threadIdx.x] = v;
+    else
+      torqp[3 * p + threadIdx.x - 3] = v;
+  }
+  const int64_t a = off[b];
+  const int64_t e = min(off[b + 1], cap);
+  for (int64_t l = a + threadIdx.x; l < e; l += PB) {
     const uint32_t n = lk_node[l];
     const int q = lk_dir[l];
     if (n >= g.ncell || q < 1 || q >= NPOP) continue;
@@ -424,23 +542,24 @@ __global__ void __launch_bounds__(PB)
     node_coords(n, g, x, y, z, row);
     const int yt = y + cy(q);
     if (g.walls_y && (yt < 0 || yt >= g.ny)) continue;
-    const uint32_t t = (uint32_t)wrap_any(x + cx(q), g.nx) + (uint32_t)g.nx * ((uint32_t)wrap_any(yt, g.ny) + (uint32_t)g.ny * (uint32_t)wrap_any(z + cz(q), g.nz));
+    const uint32_t t = (uint32_t)wrap1(x + cx(q), g.nx) + (uint32_t)g.nx * ((uint32_t)wrap1(yt, g.ny) + (uint32_t)g.ny * (uint32_t)wrap1(z + cz(q), g.nz));
     A[(uint64_t)opp(q) * g.ncell + t] = (T)lk_out[l];
   }
 }
 
-// beads_lubforce (partlib.f90:2211-2292): one thread per particle sums its
-// pair forces in the reference's order (pairs with lower ids first).  Each
-// pair is evaluated as the reference does, from the lower id's side.
-__global__ void __launch_bounds__(256)
+// beads_lubforce (partlib.f90:2211-2292): one workgroup per particle sums its
+// pair forces; each pair is evaluated as the reference does, from the lower
+// id's side (so both partners see bitwise opposite pair forces).
+__global__ void __launch_bounds__(PB)
     k_lubforce(const double* __restrict__ ypglb, int np, PGeom G, double mingap, double fpc,
                double stf0, double stf1, double* __restrict__ flubp) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double red[3][PB / 64];
+  const int a = blockIdx.x;
   if (a >= np) return;
   const double dist0 = 2.0 * G.rad + mingap;
   const double pre = (dist0 + 1e-6) * (dist0 + 1e-6);
   double fx = 0.0, fy = 0.0, fz = 0.0;
-  for (int b = 0; b < np; ++b) {
+  for (int b = threadIdx.x; b < np; b += PB) {
     if (b == a) continue;
     const int lo = a < b ? a : b, hi = a < b ? b : a;
     const double xc = ypglb[3 * lo], yc = ypglb[3 * lo + 1], zc = ypglb[3 * lo + 2];
@@ -452,30 +571,58 @@ __global__ void __launch_bounds__(256)
     if (!(d2 < pre)) continue;
     const double dist = sqrt(d2);
     const double fpd = lub_fpd(dist, G.rad, mingap, fpc, stf0, stf1);
-    if (a == lo) {
-      fx += fpd * dx;
-      fy += fpd * dy;
-      fz += fpd * dz;
-    } else {
-      fx -= fpd * dx;
-      fy -= fpd * dy;
-      fz -= fpd * dz;
-    }
+    const double sgn = (a == lo) ? 1.0 : -1.0;
+    fx += sgn * (fpd * dx);
+    fy += sgn * (fpd * dy);
+    fz += sgn * (fpd * dz);
   }
-  flubp[3 * a] = fx;
-  flubp[3 * a + 1] = fy;
-  flubp[3 * a + 2] = fz;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double v[3] = {fx, fy, fz};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v[c] += __shfl_down(v[c], o, 64);
+    if (lane == 0) red[c][w] = v[c];
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < PB / 64; ++q) s += red[threadIdx.x][q];
+    flubp[3 * a + threadIdx.x] = s;
+  }
 }
 
 struct MoveArgs {
-  double *yp, *wp, *wpp, *omgp, *omgpp, *fHIp, *flubp, *torqp, *forcep, *forcepp, *torqpp, *dwdt,
-      *domgdt, *thetap;
+  double *yp, *ypglb, *ypglbp, *wp, *wpp, *omgp, *omgpp, *fHIp, *flubp, *torqp, *forcep, *forcepp,
+      *torqpp, *dwdt, *domgdt, *thetap;
   double volp, rhog, amp, aip;
   double bfx, bfy, bfz;
 };
 
-// beads_move (partlib.f90:2295-2394): second-order Newton-Euler update.
-__global__ void __launch_bounds__(256) k_move(MoveArgs m, int np, int first) {
+__device__ __forceinline__ bool wrap_particle(double* yp, double* ypglb, double* ypglbp, int p,
+                                              const PGeom& G) {
+  const int ns[3] = {G.nx, G.ny, G.nz};
+  bool bad = false;
+  for (int c = 0; c < 3; ++c) {
+    const int q = 3 * p + c;
+    const double v = yp[q], N = (double)ns[c];
+    if (!(G.walls_y && c == 1)) {
+      if (v >= N) yp[q] = yp[q] - N;
+      if (v < 0.0) yp[q] = yp[q] + N;
+    }
+    if (!sane_coord(yp[q], ns[c])) bad = true;
+    ypglbp[q] = ypglb[q];
+    ypglb[q] = yp[q];
+  }
+  return bad;
+}
+
+// beads_move (partlib.f90:2295-2394): second-order Newton-Euler update.  With
+// `wrap` set the periodic wrap of beads_redistribute (partlib.f90:2600-2626)
+// follows in the same thread.
+__global__ void __launch_bounds__(256)
+    k_move(MoveArgs m, int np, int first, int wrap, PGeom G, int32_t* __restrict__ status) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   const double dt = 1.0;
@@ -517,6 +664,7 @@ __global__ void __launch_bounds__(256) k_move(MoveArgs m, int np, int first) {
     m.forcepp[q] = m.forcep[q];
     m.torqpp[q] = m.torqp[q];
   }
+  if (wrap && wrap_particle(m.yp, m.ypglb, m.ypglbp, p, G)) atomicAdd(&status[ST_NONFINITE], 1);
 }
 
 // beads_redistribute, one-rank content (partlib.f90:2600-2626): periodic wrap,
@@ -526,40 +674,31 @@ __global__ void __launch_bounds__(256)
            PGeom G, int32_t* __restrict__ status) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  const int ns[3] = {G.nx, G.ny, G.nz};
-  bool bad = false;
-  for (int c = 0; c < 3; ++c) {
-    const int q = 3 * p + c;
-    const double v = yp[q], N = (double)ns[c];
-    if (!(G.walls_y && c == 1)) {
-      if (v >= N) yp[q] = yp[q] - N;
-      if (v < 0.0) yp[q] = yp[q] + N;
-    }
-    if (!sane_coord(yp[q], ns[c])) bad = true;
-    ypglbp[q] = ypglb[q];
-    ypglb[q] = yp[q];
-  }
-  if (bad) atomicAdd(&status[ST_NONFINITE], 1);
+  if (wrap_particle(yp, ypglb, ypglbp, p, G)) atomicAdd(&status[ST_NONFINITE], 1);
 }
 
-// ovlpchk (partlib.f90:118-171): flag the lower id of any pair closer than 2r.
-__global__ void __launch_bounds__(256)
+// ovlpchk (partlib.f90:118-171): one workgroup per particle flags it if any
+// later particle is closer than 2r.
+__global__ void __launch_bounds__(PB)
     k_overlap(const double* __restrict__ ypglb, int np, PGeom G, int istep,
               int32_t* __restrict__ status) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= np - 1) return;
+  __shared__ int hit;
+  const int a = blockIdx.x;
+  if (a >= np) return;
+  if (threadIdx.x == 0) hit = 0;
+  __syncthreads();
   const double xc = ypglb[3 * a], yc = ypglb[3 * a + 1], zc = ypglb[3 * a + 2];
-  for (int b = a + 1; b < np; ++b) {
+  for (int b = a + 1 + threadIdx.x; b < np; b += PB) {
     const double xb = near_image(ypglb[3 * b], xc, G.nx);
     const double yb = G.walls_y ? ypglb[3 * b + 1] : near_image(ypglb[3 * b + 1], yc, G.ny);
     const double zb = near_image(ypglb[3 * b + 2], zc, G.nz);
     const double dist = sqrt((xc - xb) * (xc - xb) + (yc - yb) * (yc - yb) + (zc - zb) * (zc - zb));
-    const double hgap = dist - 2.0 * G.rad;
-    if (hgap < 0.0) {
-      atomicAdd(&status[ST_OVERLAP], 1);
-      atomicCAS(&status[ST_OVERLAP_STEP], 0, istep);
-      break;
-    }
+    if (dist - 2.0 * G.rad < 0.0) hit = 1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && hit) {
+    atomicAdd(&status[ST_OVERLAP], 1);
+    atomicCAS(&status[ST_OVERLAP_STEP], 0, istep);
   }
 }
 
@@ -568,34 +707,59 @@ struct FillArgs {
   const int32_t* isn_old;
   const uint32_t *mask_new, *mask_old;
   double rad, rho0, ww0, ww1, ww2;
+  uint64_t* list;       // (particle << 32) | node of every node to refill
+  int32_t* count;       // list length (reset by k_scan_counts of the links pass)
+  int64_t cap;
 };
 
-// beads_filling (partlib.f90:2846-3094): nodes solid in the previous map and
-// fluid in the new one get the equilibrium at the rigid-body velocity and the
-// mean density of their fluid neighbours, plus the non-equilibrium part of
-// the neighbour along the back-traced surface normal.  One workgroup scans
-// the previous-step box of particle p.
-template <typename T, bool WALLS>
+// beads_filling, pass 1 (partlib.f90:2846-2849): find the nodes that were
+// solid for particle p in the previous map and are fluid in the new one.
+// One workgroup scans one segment of the previous-step box.
 __global__ void __launch_bounds__(PB)
-    k_filling(T* __restrict__ A, Grid g, PGeom G, FillArgs fa, int np, int32_t* __restrict__ status) {
+    k_fill_find(PGeom G, FillArgs fa, int np, int nseg, int32_t* __restrict__ status) {
   __shared__ BoxInfo B;
-  const int p = blockIdx.x;
+  const int b = blockIdx.x;
+  const int p = b / nseg, s = b - p * nseg;
   if (p >= np) return;
-  if (threadIdx.x == 0) box_info(fa.mpos, p, G, B);
-  __syncthreads();
+  box_info(fa.mpos, p, G, B);
   if (!B.ok) return;
-  const int nbox = B.ax.cnt * B.ay.cnt * B.az.cnt;
   int nfill = 0;
-  for (int t = threadIdx.x; t < nbox; t += PB) {
-    const int tx = t % B.ax.cnt, rest = t / B.ax.cnt, ty = rest % B.ay.cnt, tz = rest / B.ay.cnt;
-    const int k = wrap_any(B.az.lo + tz, G.nz);
-    const int j = G.walls_y ? B.ay.lo + ty : wrap_any(B.ay.lo + ty, G.ny);
-    const int i = wrap_any(B.ax.lo + tx, G.nx);
-    if (j < 0 || j >= G.ny) continue;
+  walk_segment(B, G, s, [&](bool act, int i, int j, int k) {
+    LinkGeom L;
+    if (!act || !node_geom(B, G, i, j, k, L) || L.rr0 > G.rsolid2) return;
     const uint32_t n = (uint32_t)i + (uint32_t)G.nx * ((uint32_t)j + (uint32_t)G.ny * (uint32_t)k);
-    if (fa.isn_old[n] != p + 1 || bit_at(fa.mask_new, n)) continue;
+    if (fa.isn_old[n] != p + 1 || bit_at(fa.mask_new, n)) return;
     ++nfill;
-    const int id = p;
+    const int slot = atomicAdd(fa.count, 1);
+    if (slot < fa.cap) fa.list[slot] = ((uint64_t)(uint32_t)p << 32) | n;
+  });
+  if (nfill) atomicAdd(&status[ST_REFILL], nfill);
+}
+
+// beads_filling, pass 2 (partlib.f90:2851-3089): one wavefront per listed
+// node; lane l examines neighbour l+1 (fluid in both maps?  its density; the
+// non-equilibrium part if it lies along the back-traced normal), then lane i
+// writes population i of the node: equilibrium at the rigid-body velocity
+// (new w, omega; old centre) and the neighbours' mean density, plus the
+// non-equilibrium part.  Written into the pull sources of the node.
+template <typename T, bool WALLS>
+__global__ void __launch_bounds__(256)
+    k_fill_apply(T* __restrict__ A, Grid g, FillArgs fa) {
+  __shared__ double fneq[4][NPOP];
+  __shared__ double srho[4][NPOP];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t nwaves = (int64_t)gridDim.x * 4;
+  const int64_t total = min((int64_t)*fa.count, fa.cap);
+  // the trip count is the same for every wavefront of the block (barriers inside)
+  for (int64_t base = (int64_t)blockIdx.x * 4; base < total; base += nwaves) {
+    const int64_t e = base + w;
+    const bool active = e < total;
+    const uint64_t ent = active ? fa.list[e] : 0ull;
+    const int id = (int)(ent >> 32);
+    const uint32_t n = (uint32_t)(ent & 0xffffffffu);
+    int i, j, k;
+    uint32_t row;
+    node_coords(n, g, i, j, k, row);
     const double xp = (double)(i + 1) - 0.5, yp = (double)(j + 1) - 0.5, zp = (double)(k + 1) - 0.5;
     const double xc = near_image(fa.ypglbp[3 * id] * 1.0, xp, g.nx);
     const double yc = WALLS ? fa.ypglbp[3 * id + 1] : near_image(fa.ypglbp[3 * id + 1] * 1.0, yp, g.ny);
@@ -607,59 +771,66 @@ __global__ void __launch_bounds__(PB)
     const double o2 = -0.5 * (fa.omgp[3 * id + 1] * 1.0 + fa.omgpp[3 * id + 1] * 1.0);
     const double o3 = -0.5 * (fa.omgp[3 * id + 2] * 1.0 + fa.omgpp[3 * id + 2] * 1.0);
     const int ipmx = refill_direction(xp, yp, zp, xc, yc, zc, w1, w2, w3, o1, o2, o3, fa.rad);
-    double fn[NPOP];
-    {
-      const int a = i + cx(ipmx), bb = j + cy(ipmx), c = k + cz(ipmx);
+    // lane l < 18: neighbour d = l + 1
+    bool fluid = false;
+    if (active && lane < NPOP - 1) {
+      const int d = lane + 1;
+      const int a = i + cx(d), bb = j + cy(d), c = k + cz(d);
       const bool out = WALLS && (bb < 0 || bb >= g.ny);
-      const uint32_t q = out ? n : (uint32_t)wrap_any(a, g.nx) + (uint32_t)g.nx * ((uint32_t)wrap_any(bb, g.ny) + (uint32_t)g.ny * (uint32_t)wrap_any(c, g.nz));
-      if (!out && !bit_at(fa.mask_new, q) && !bit_at(fa.mask_old, q)) {
-        int qx, qy, qz;
-        uint32_t row;
-        node_coords(q, g, qx, qy, qz, row);
-        double f9[NPOP];
-        load_pulled<T, double, WALLS>(A, g, qx, qy, qz, q, f9);
-        double rho9 = 0.0, u9 = 0.0, v9 = 0.0, w9 = 0.0;
-        for (int ip = 0; ip < NPOP; ++ip) {
-          rho9 = rho9 + f9[ip];
-          u9 = u9 + (double)cx(ip) * f9[ip];
-          v9 = v9 + (double)cy(ip) * f9[ip];
-          w9 = w9 + (double)cz(ip) * f9[ip];
+      if (!out) {
+        const uint32_t q = (uint32_t)wrap1(a, g.nx) + (uint32_t)g.nx * ((uint32_t)wrap1(bb, g.ny) + (uint32_t)g.ny * (uint32_t)wrap1(c, g.nz));
+        fluid = !bit_at(fa.mask_new, q) && !bit_at(fa.mask_old, q);
+        double s = 0.0;
+        if (fluid) {
+          int qx, qy, qz;
+          uint32_t r2;
+          node_coords(q, g, qx, qy, qz, r2);
+          double f9[NPOP];
+          load_pulled<T, double, WALLS>(A, g, qx, qy, qz, q, f9);
+          for (int ii = 0; ii < NPOP; ++ii) s = s + f9[ii];
+          if (d == ipmx) {
+            double rho9 = 0.0, u9 = 0.0, v9 = 0.0, w9 = 0.0;
+            for (int ip = 0; ip < NPOP; ++ip) {
+              rho9 = rho9 + f9[ip];
+              u9 = u9 + (double)cx(ip) * f9[ip];
+              v9 = v9 + (double)cy(ip) * f9[ip];
+              w9 = w9 + (double)cz(ip) * f9[ip];
+            }
+            double feq[NPOP];
+            feq_point(u9, v9, w9, rho9, fa.ww0, fa.ww1, fa.ww2, feq);
+            for (int ip = 0; ip < NPOP; ++ip) fneq[w][ip] = f9[ip] - feq[ip];
+          }
         }
-        double feq[NPOP];
-        feq_point(u9, v9, w9, rho9, fa.ww0, fa.ww1, fa.ww2, feq);
-        for (int ip = 0; ip < NPOP; ++ip) fn[ip] = f9[ip] - feq[ip];
+        srho[w][lane] = s;
       } else {
-        for (int ip = 0; ip < NPOP; ++ip) fn[ip] = 0.0;
+        srho[w][lane] = 0.0;
       }
     }
-    int nghb = 0;
-    double rho9 = 0.0;
-    for (int ip = 1; ip < NPOP; ++ip) {
-      const int a = i + cx(ip), bb = j + cy(ip), c = k + cz(ip);
-      if (WALLS && (bb < 0 || bb >= g.ny)) continue;
-      const uint32_t q = (uint32_t)wrap_any(a, g.nx) + (uint32_t)g.nx * ((uint32_t)wrap_any(bb, g.ny) + (uint32_t)g.ny * (uint32_t)wrap_any(c, g.nz));
-      if (!bit_at(fa.mask_new, q) && !bit_at(fa.mask_old, q)) {
-        ++nghb;
-        int qx, qy, qz;
-        uint32_t row;
-        node_coords(q, g, qx, qy, qz, row);
-        double f9[NPOP];
-        load_pulled<T, double, WALLS>(A, g, qx, qy, qz, q, f9);
-        for (int ii = 0; ii < NPOP; ++ii) rho9 = rho9 + f9[ii];
-      }
+    const uint64_t fl = __ballot(fluid);
+    const bool have_neq = (fl >> (ipmx - 1)) & 1ull;
+    __syncthreads();
+    if (active && lane < NPOP) {
+      const int nghb = __popcll(fl);
+      double rho9 = 0.0;
+      for (int l = 0; l < NPOP - 1; ++l)
+        if ((fl >> l) & 1ull) rho9 = rho9 + srho[w][l];
+      rho9 = (nghb > 0) ? rho9 / (double)nghb : fa.rho0;
+      const double xx0 = xp - xc, yy0 = yp - yc, zz0 = zp - zc;
+      const double W1 = fa.wp[3 * id], W2 = fa.wp[3 * id + 1], W3 = fa.wp[3 * id + 2];
+      const double O1 = fa.omgp[3 * id], O2 = fa.omgp[3 * id + 1], O3 = fa.omgp[3 * id + 2];
+      const double u9 = W1 + O2 * zz0 - O3 * yy0;
+      const double v9 = W2 + O3 * xx0 - O1 * zz0;
+      const double w9 = W3 + O1 * yy0 - O2 * xx0;
+      const double usqr = 1.5 * (u9 * u9 + v9 * v9 + w9 * w9);
+      const int ip = lane;
+      const double G = (double)cx(ip) * u9 + (double)cy(ip) * v9 + (double)cz(ip) * w9;
+      const double feq = ip == 0 ? fa.ww0 * (rho9 - usqr)
+                                 : (ip <= 6 ? fa.ww1 : fa.ww2) * (rho9 + 3.0 * G + 4.5 * G * G - usqr);
+      const double fn = have_neq ? fneq[w][ip] : 0.0;
+      store_phase<T, WALLS>(A, g, ip, i, j, k, n, true, fn + feq);
     }
-    rho9 = (nghb > 0) ? rho9 / (double)nghb : fa.rho0;
-    const double xx0 = xp - xc, yy0 = yp - yc, zz0 = zp - zc;
-    const double W1 = fa.wp[3 * id], W2 = fa.wp[3 * id + 1], W3 = fa.wp[3 * id + 2];
-    const double O1 = fa.omgp[3 * id], O2 = fa.omgp[3 * id + 1], O3 = fa.omgp[3 * id + 2];
-    const double u9 = W1 + O2 * zz0 - O3 * yy0;
-    const double v9 = W2 + O3 * xx0 - O1 * zz0;
-    const double w9 = W3 + O1 * yy0 - O2 * xx0;
-    double feq[NPOP];
-    feq_point(u9, v9, w9, rho9, fa.ww0, fa.ww1, fa.ww2, feq);
-    for (int ip = 0; ip < NPOP; ++ip) store_phase<T, WALLS>(A, g, ip, i, j, k, n, true, fn[ip] + feq[ip]);
+    __syncthreads();
   }
-  if (nfill) atomicAdd(&status[ST_REFILL], nfill);
 }
 
 // trilinear velocity at the particle centres (initpartvel,
@@ -699,13 +870,14 @@ __global__ void k_maps_dense(Grid g, const int32_t* __restrict__ isn, const uint
   is[n] = (isn != nullptr && isn[n] > 0) ? isn[n] : -1;
 }
 
-__global__ void k_links_dense(Grid g, const int64_t* __restrict__ off, int64_t cap,
+__global__ void k_links_dense(Grid g, int nseg, const int64_t* __restrict__ off, int64_t cap,
                               const uint32_t* __restrict__ lk_node, const uint8_t* __restrict__ lk_dir,
                               const double* __restrict__ lk_alpha, int32_t* __restrict__ ilink,
                               int32_t* __restrict__ mlink, double* __restrict__ alink) {
-  const int p = blockIdx.x;
-  const int64_t a = off[p], b = min(off[p + 1], cap);
-  for (int64_t l = a + threadIdx.x; l < b; l += blockDim.x) {
+  const int b = blockIdx.x;
+  const int p = b / nseg;
+  const int64_t a = off[b], e = min(off[b + 1], cap);
+  for (int64_t l = a + threadIdx.x; l < e; l += blockDim.x) {
     if (lk_node[l] >= g.ncell || lk_dir[l] < 1 || lk_dir[l] >= NPOP) continue;
     const uint64_t d = (uint64_t)(lk_dir[l] - 1) + 18ull * lk_node[l];
     ilink[d] = 1;
@@ -724,28 +896,34 @@ static PGeom pgeom(const Ctx& c) {
   G.rad = c.p.rad;
   G.radp2 = c.p.rad * 1.0 + 2.0;
   G.r2 = (c.p.rad * 1.0) * (c.p.rad * 1.0);
+  G.rlink2 = (c.p.rad + 1.5) * (c.p.rad + 1.5);
+  G.rsolid2 = (c.p.rad + 0.5) * (c.p.rad + 0.5);
   return G;
 }
 
-hipError_t launch_clear_map(Ctx& c, int target) {
-  Particles& P = c.pt;
-  if (P.map_np[target] > 0)
-    hipLaunchKernelGGL(k_clear_map, dim3(P.map_np[target]), dim3(PB), 0, c.stream, P.map_pos[target],
-                       P.map_np[target], pgeom(c), P.isn[target], P.mask[target]);
-  return hipGetLastError();
+int segments_per_particle(double rad) {
+  // the widest candidate range axis_range() can return, in z-slabs
+  const double radp2 = rad + 2.0;
+  const int maxcnt = (int)ceil(2.0 * radp2 + 1.0) + 4;
+  return (maxcnt + SLAB - 1) / SLAB;
 }
 
-hipError_t launch_links(Ctx& c, int target) {
+hipError_t launch_links(Ctx& c, int target, int overlap_check, int istep) {
   Particles& P = c.pt;
   const PGeom G = pgeom(c);
-  hipLaunchKernelGGL(k_links_count, dim3(P.np), dim3(PB), 0, c.stream, P.ypglb, P.np, G, P.lk_cnt,
-                     P.sol_cnt, c.d_status);
-  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, c.stream, P.lk_cnt, P.sol_cnt, P.np,
-                     P.lk_off, P.lk_cap, c.d_status);
-  hipLaunchKernelGGL(k_links_write, dim3(P.np), dim3(PB), 0, c.stream, P.ypglb, P.np, G, P.lk_off,
-                     P.lk_cap, P.isn[target], P.mask[target], P.lk_node, P.lk_dir, P.lk_alpha,
-                     c.d_status);
-  hipMemcpyAsync(P.map_pos[target], P.ypglb, sizeof(double) * 3 * P.np, hipMemcpyDeviceToDevice, c.stream);
+  const int nsegs = P.np * P.nseg;
+  const int clr_np = P.map_np[target];
+  const int nblk = (P.np > clr_np ? P.np : clr_np) * P.nseg;
+  hipLaunchKernelGGL(k_links_count, dim3(nblk), dim3(PB), 0, c.stream, P.ypglb, P.np, P.nseg, G,
+                     P.map_pos[target], clr_np, P.isn[target], P.mask[target], P.lk_cnt, P.sol_cnt,
+                     overlap_check, istep, c.d_status);
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, c.stream, P.lk_cnt, P.sol_cnt, nsegs,
+                     P.lk_off, P.lk_cap, c.d_status, P.fill_count);
+  hipLaunchKernelGGL(k_links_write, dim3(nsegs), dim3(PB), 0, c.stream, P.ypglb, P.np, P.nseg, G,
+                     P.lk_off, P.lk_cap, P.isn[target], P.mask[target], P.map_pos[target],
+                     P.lk_node, P.lk_dir);
+  hipLaunchKernelGGL(k_link_alpha, dim3(nsegs), dim3(PB), 0, c.stream, P.ypglb, P.np, P.nseg, G,
+                     P.lk_off, P.lk_cap, P.lk_node, P.lk_dir, P.lk_alpha, c.d_status);
   P.map_np[target] = P.np;
   return hipGetLastError();
 }
@@ -754,11 +932,12 @@ template <typename T>
 static void bb_dispatch(Ctx& c) {
   Particles& P = c.pt;
   T* A = (T*)c.pop[c.cur];
-  hipLaunchKernelGGL((k_bounce_back<T>), dim3(P.np), dim3(PB), 0, c.stream, A, c.g, pgeom(c),
-                     P.mask[P.map_cur], P.lk_off, P.lk_cap, P.lk_node, P.lk_dir, P.lk_alpha, P.lk_out,
-                     P.ypglb, P.wp, P.omgp, c.p.ww1, c.p.ww2, P.fHIp, P.torqp, c.d_status);
-  hipLaunchKernelGGL((k_bb_scatter<T>), dim3(P.np), dim3(PB), 0, c.stream, A, c.g, P.lk_off, P.lk_cap,
-                     P.lk_node, P.lk_dir, P.lk_out);
+  const int nsegs = P.np * P.nseg;
+  hipLaunchKernelGGL((k_bounce_back<T>), dim3(nsegs), dim3(PB), 0, c.stream, A, c.g, pgeom(c),
+                     P.mask[P.map_cur], P.nseg, P.lk_off, P.lk_cap, P.lk_node, P.lk_dir, P.lk_alpha,
+                     P.lk_out, P.ypglb, P.wp, P.omgp, c.p.ww1, c.p.ww2, P.seg_ft, c.d_status);
+  hipLaunchKernelGGL((k_bb_scatter<T>), dim3(nsegs), dim3(PB), 0, c.stream, A, c.g, P.nseg, P.lk_off,
+                     P.lk_cap, P.lk_node, P.lk_dir, P.lk_out, P.seg_ft, P.fHIp, P.torqp);
 }
 
 hipError_t launch_bounce_back(Ctx& c) {
@@ -772,21 +951,23 @@ hipError_t launch_bounce_back(Ctx& c) {
 hipError_t launch_lubforce(Ctx& c) {
   Particles& P = c.pt;
   const double fpc = c.p.volp * c.p.rho0 * 9.8 / c.p.gscale;
-  hipLaunchKernelGGL(k_lubforce, dim3((P.np + 255) / 256), dim3(256), 0, c.stream, P.ypglb, P.np,
-                     pgeom(c), c.p.mingap, fpc, c.p.stf0, c.p.stf1, P.flubp);
+  hipLaunchKernelGGL(k_lubforce, dim3(P.np), dim3(PB), 0, c.stream, P.ypglb, P.np, pgeom(c),
+                     c.p.mingap, fpc, c.p.stf0, c.p.stf1, P.flubp);
   return hipGetLastError();
 }
 
-hipError_t launch_move(Ctx& c, int first) {
+hipError_t launch_move(Ctx& c, int first, int wrap) {
   Particles& P = c.pt;
   MoveArgs m;
-  m.yp = P.yp; m.wp = P.wp; m.wpp = P.wpp; m.omgp = P.omgp; m.omgpp = P.omgpp;
+  m.yp = P.yp; m.ypglb = P.ypglb; m.ypglbp = P.ypglbp;
+  m.wp = P.wp; m.wpp = P.wpp; m.omgp = P.omgp; m.omgpp = P.omgpp;
   m.fHIp = P.fHIp; m.flubp = P.flubp; m.torqp = P.torqp; m.forcep = P.forcep;
   m.forcepp = P.forcepp; m.torqpp = P.torqpp; m.dwdt = P.dwdt; m.domgdt = P.domgdt;
   m.thetap = P.thetap;
   m.volp = c.p.volp; m.rhog = c.p.rhog; m.amp = c.p.amp; m.aip = c.p.aip;
   m.bfx = c.p.body_force[0]; m.bfy = c.p.body_force[1]; m.bfz = c.p.body_force[2];
-  hipLaunchKernelGGL(k_move, dim3((P.np + 255) / 256), dim3(256), 0, c.stream, m, P.np, first);
+  hipLaunchKernelGGL(k_move, dim3((P.np + 255) / 256), dim3(256), 0, c.stream, m, P.np, first, wrap,
+                     pgeom(c), c.d_status);
   return hipGetLastError();
 }
 
@@ -794,8 +975,8 @@ hipError_t launch_redistribute(Ctx& c, int istep) {
   Particles& P = c.pt;
   hipLaunchKernelGGL(k_wrap, dim3((P.np + 255) / 256), dim3(256), 0, c.stream, P.yp, P.ypglb,
                      P.ypglbp, P.np, pgeom(c), c.d_status);
-  hipLaunchKernelGGL(k_overlap, dim3((P.np + 255) / 256), dim3(256), 0, c.stream, P.ypglb, P.np,
-                     pgeom(c), istep, c.d_status);
+  hipLaunchKernelGGL(k_overlap, dim3(P.np), dim3(PB), 0, c.stream, P.ypglb, P.np, pgeom(c), istep,
+                     c.d_status);
   return hipGetLastError();
 }
 
@@ -810,19 +991,25 @@ hipError_t launch_filling(Ctx& c) {
   fa.mask_new = P.mask[P.map_cur];
   fa.mask_old = P.mask[old];
   fa.rad = c.p.rad; fa.rho0 = c.p.rho0; fa.ww0 = c.p.ww0; fa.ww1 = c.p.ww1; fa.ww2 = c.p.ww2;
+  fa.list = P.fill_list;
+  fa.count = P.fill_count;
+  fa.cap = P.fill_cap;
   const int np = P.map_np[old];
+  hipLaunchKernelGGL(k_fill_find, dim3(np * P.nseg), dim3(PB), 0, c.stream, pgeom(c), fa, np, P.nseg,
+                     c.d_status);
+  const dim3 grid(1024), blk(256);
   if (c.prec == MFC_PREC_FP64) {
     double* A = (double*)c.pop[c.cur];
     if (c.g.walls_y)
-      hipLaunchKernelGGL((k_filling<double, true>), dim3(np), dim3(PB), 0, c.stream, A, c.g, pgeom(c), fa, np, c.d_status);
+      hipLaunchKernelGGL((k_fill_apply<double, true>), grid, blk, 0, c.stream, A, c.g, fa);
     else
-      hipLaunchKernelGGL((k_filling<double, false>), dim3(np), dim3(PB), 0, c.stream, A, c.g, pgeom(c), fa, np, c.d_status);
+      hipLaunchKernelGGL((k_fill_apply<double, false>), grid, blk, 0, c.stream, A, c.g, fa);
   } else {
     float* A = (float*)c.pop[c.cur];
     if (c.g.walls_y)
-      hipLaunchKernelGGL((k_filling<float, true>), dim3(np), dim3(PB), 0, c.stream, A, c.g, pgeom(c), fa, np, c.d_status);
+      hipLaunchKernelGGL((k_fill_apply<float, true>), grid, blk, 0, c.stream, A, c.g, fa);
     else
-      hipLaunchKernelGGL((k_filling<float, false>), dim3(np), dim3(PB), 0, c.stream, A, c.g, pgeom(c), fa, np, c.d_status);
+      hipLaunchKernelGGL((k_fill_apply<float, false>), grid, blk, 0, c.stream, A, c.g, fa);
   }
   return hipGetLastError();
 }
@@ -845,8 +1032,8 @@ hipError_t launch_maps_to_host_layout(Ctx& c, int which, int32_t* ib, int32_t* i
 
 hipError_t launch_links_dense(Ctx& c, int32_t* ilink, int32_t* mlink, double* alink) {
   Particles& P = c.pt;
-  hipLaunchKernelGGL(k_links_dense, dim3(P.np), dim3(256), 0, c.stream, c.g, P.lk_off, P.lk_cap,
-                     P.lk_node, P.lk_dir, P.lk_alpha, ilink, mlink, alink);
+  hipLaunchKernelGGL(k_links_dense, dim3(P.np * P.nseg), dim3(256), 0, c.stream, c.g, P.nseg, P.lk_off,
+                     P.lk_cap, P.lk_node, P.lk_dir, P.lk_alpha, ilink, mlink, alink);
   return hipGetLastError();
 }
 

@@ -139,14 +139,15 @@ def test_moving_spheres_refill_nodes(oracle_lib):
     lat.prt_set("ypglb", pos)
     lat.prt_set("wp", vel)
     lat.beads_links()
-    refills = 0
+    refills_orc = 0
     for it in range(2, 14):     # past the release step: keep the given velocities
         orc.S.istep = it
         orc.call("timestep")
+        refills_orc += int(((orc.ibnodes0 > 0) & (orc.ibnodes < 0)).sum())
         lat.step(it, 1)
         lat.sync()
-        refills += lat.status()["refill"]
-    assert refills > 0
+    assert refills_orc > 0
+    assert lat.status()["refill"] == refills_orc   # cumulative counter
     assert rel(lat.download_f(), orc.f) < TOL64
     assert rel(lat.prt_get("ypglb"), orc.ypglb) < TOL64
     assert rel(lat.prt_get("fHIp"), orc.fHIp) < 1e-11
