@@ -96,7 +96,8 @@ double ran1(int32_t* idum, int32_t* iy, int32_t* iv) {
 }
 
 inline int mix(int i, int j, int k) { return (i - 1) + 6 * ((j - 1) + 5 * (k - 1)); }
-inline double kx_of(int i) { return (double)(i / 2); }
+// kx of the (re, im) slot i = 1..6: both slots of a pair carry the pair's kx
+inline double kx_of(int i) { return (double)((i - 1) / 2); }
 inline double kyz_of(int j) { return j <= 3 ? (double)(j - 1) : (double)(j - 6); }
 
 // kx = 0 plane made Hermitian so the physical field is real

@@ -90,7 +90,9 @@ double orc_ranf(int *idum, int *iy, int *iv) {
 /* mode-array index: a(i,j,k) with dims (6,5,5), 1-based in the reference */
 #define AIX(i, j, k) (((i) - 1) + 6 * (((j) - 1) + 5 * ((k) - 1)))
 
-static double kx_of(int i) { return (double)(i / 2); }          /* kxr, i=1..6 */
+/* kxr of the (re, im) slot i = 1..6: both slots of a pair carry the pair's kx
+ * (kxr(i) = int(i/2) set for odd i and copied to i+1, para.f90:260-263) */
+static double kx_of(int i) { return (double)((i - 1) / 2); }
 static double kyz_of(int j) { return j <= 3 ? (double)(j - 1) : (double)(j - 6); } /* kyr/kzr */
 
 /* FORCING, spectral part: OU update of the low modes, solenoidal projection,

@@ -1,0 +1,474 @@
+"""Pins of the C oracle by known answers, closed forms and invariants.
+
+What pins which oracle function (the reference has no tests or fixtures of
+its own; oracle/_ref pins collision_MRT, macrovar, rhoupdat and para bit for
+bit, see test_oracle_ref.py):
+  orc_ranf           Park-Miller minimal-standard known value; shuffle structure
+  orc_force_field    numpy.fft.irfftn of the same sparse spectrum (non-cubic box)
+  orc_forcing_modes  solenoidal, Hermitian kx=0 plane, zero mean, shell cut-off
+  orc_streaming      numpy.roll (periodic), brute-force wall rule
+  orc_collision_mrt  mass/momentum balance; BGK limit (MRT type 2) closed form
+  orc_initpop/feqpnt equilibrium moments (density, momentum, momentum flux)
+  orc_beads_links    brute-force solid set and link set; wall point on the sphere
+  orc_beads_collision  rest/uniform-density force zero; moving-sphere force closed form
+  orc_beads_move     constant force/torque: uniformly accelerated motion, exact
+  orc_beads_lubforce Newton's third law, line of centres, cut-off continuity
+  orc_beads_filling  uniform equilibrium fluid: refill = feq(rho, rigid-body u)
+  orc_initpartvel    trilinear interpolation exact on a linear field
+  orc_initpartpos    placement has no overlaps (with min gap); seeded repeatably
+  whole step         Taylor-Green shear-wave decay rate nu k^2 with nu = (1/s9 - 1/2)/3
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from multiphase_flow_codes_amd.params import CIX, CIY, CIZ, LBMParams
+
+CX = np.array(CIX)
+CY = np.array(CIY)
+CZ = np.array(CIZ)
+W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+
+
+def _orc(prm, **kw):
+    from oracle.pyoracle import Oracle
+    return Oracle(prm, **kw)
+
+
+def feq_np(rho, u):
+    """deviation equilibrium w_i (rho + 3 c.u + 4.5 (c.u)^2 - 1.5 u^2)"""
+    cu = CX * u[0] + CY * u[1] + CZ * u[2]
+    return W * (rho + 3 * cu + 4.5 * cu * cu - 1.5 * float(np.dot(u, u)))
+
+
+# ---------------------------------------------------------------- RNG ------
+def test_ranf_lcg_known_value(oracle_lib):
+    """Park & Miller (1988): from seed 1 the 10000th state of the minimal
+    standard generator is 1043618065.  ran1's reseed consumes 40 states."""
+    idum, iy = C.c_int(-1), C.c_int(0)
+    iv = (C.c_int * 32)()
+    for _ in range(10000 - 40):
+        oracle_lib.orc_ranf(C.byref(idum), C.byref(iy), iv)
+    assert idum.value == 1043618065
+
+
+def test_ranf_shuffle_structure(oracle_lib):
+    """Bays-Durham: every output is AM * (an LCG state produced earlier and
+    parked in a 32-slot table), so the delays are >= 1 with mean ~ 32; all
+    outputs distinct, in (0, 1), roughly uniform."""
+    idum, iy = C.c_int(-12345), C.c_int(0)
+    iv = (C.c_int * 32)()
+    states, outs = [], []
+    for _ in range(5000):
+        r = oracle_lib.orc_ranf(C.byref(idum), C.byref(iy), iv)
+        states.append(idum.value)
+        outs.append(r)
+    AM = 1.0 / 2147483647
+    when = {s: k for k, s in enumerate(states)}
+    outs = np.array(outs)
+    assert np.all((outs > 0) & (outs < 1))
+    assert len(set(outs.tolist())) == len(outs)
+    delays = []
+    for k, r in enumerate(outs[200:], 200):
+        s = int(round(r / AM))
+        assert s in when and when[s] < k      # an earlier state of the LCG
+        delays.append(k - when[s])
+    assert 25 < np.mean(delays) < 40
+    assert abs(outs.mean() - 0.5) < 0.02
+
+
+# ------------------------------------------------------------- forcing -----
+def _hermitian_modes(rng):
+    """Random (6,5,5) mode arrays with the kx = 0 plane Hermitian, as FORCING
+    leaves them (gaussian.f90:284-306)."""
+    a = rng.normal(size=(5, 5, 6))   # [k][j][i] == Fortran a(i,j,k)
+    for k in range(1, 6):
+        kk = 1 if k == 1 else 7 - k
+        for j in (2, 3):
+            jj = 7 - j
+            a[kk - 1, jj - 1, 0] = a[k - 1, j - 1, 0]
+            a[kk - 1, jj - 1, 1] = -a[k - 1, j - 1, 1]
+    for k in (2, 3):
+        kk = 7 - k
+        a[kk - 1, 0, 0] = a[k - 1, 0, 0]
+        a[kk - 1, 0, 1] = -a[k - 1, 0, 1]
+    a[0, 0, 0:2] = 0.0
+    return a
+
+
+def test_force_field_matches_irfftn(oracle_lib):
+    """Direct mode evaluation == unnormalised inverse real FFT of the sparse
+    spectrum (FFTW-2 backward conventions): axes x, y, z distinct."""
+    rng = np.random.default_rng(0)
+    prm = LBMParams(nx=12, ny=10, nz=8, visc=0.01)
+    o = _orc(prm)
+    modes = [_hermitian_modes(rng) for _ in range(3)]
+    o.force_field(*modes)
+    kyz = [0, 1, 2, -2, -1]
+    for comp, out in zip(modes, (o.fx, o.fy, o.fz)):
+        S = np.zeros((prm.nz, prm.ny, prm.nx // 2 + 1), dtype=complex)
+        for k in range(5):
+            for j in range(5):
+                for kx in range(3):
+                    S[kyz[k] % prm.nz, kyz[j] % prm.ny, kx] += comp[k, j, 2 * kx] + 1j * comp[k, j, 2 * kx + 1]
+        ref = np.fft.irfftn(S, s=(prm.nz, prm.ny, prm.nx), axes=(0, 1, 2)) * prm.nz * prm.ny * prm.nx
+        assert np.max(np.abs(out - ref)) < 1e-12 * np.max(np.abs(ref))
+
+
+def test_forcing_modes_structure(oracle_lib):
+    prm = LBMParams(nx=16, ny=16, nz=16, visc=0.01)
+    o = _orc(prm)
+    for _ in range(5):
+        o.call("forcing_modes")
+    kx = np.array([0, 0, 1, 1, 2, 2])
+    kyz = np.array([0, 1, 2, -2, -1])
+    a = [np.array(m).reshape(5, 5, 6) for m in o.modes()]
+    kvec = np.meshgrid(kyz, kyz, kx, indexing="ij")   # [k][j][i]: kz, ky, kx
+    kzg, kyg, kxg = kvec
+    div = kxg * a[0] + kyg * a[1] + kzg * a[2]
+    assert np.max(np.abs(div)) < 1e-14 * max(np.max(np.abs(a[0])), 1e-300)
+    rad = np.sqrt(kxg ** 2 + kyg ** 2 + kzg ** 2)
+    for m in a:
+        assert np.all(m[rad >= math.sqrt(8.0)] == 0.0)
+        assert m[0, 0, 0] == 0.0 and m[0, 0, 1] == 0.0
+        assert np.any(m != 0.0)
+    # the physical field is real and mean-free: check via the field itself
+    o.force_field(*a)
+    for fld in (o.fx, o.fy, o.fz):
+        assert abs(fld.mean()) < 1e-13 * np.max(np.abs(fld))
+
+
+# ----------------------------------------------------------- streaming -----
+def test_streaming_is_periodic_shift(oracle_lib):
+    rng = np.random.default_rng(1)
+    prm = LBMParams(nx=7, ny=5, nz=6, visc=0.01)
+    o = _orc(prm)
+    o.f[:] = rng.normal(size=o.f.shape)
+    f0 = o.f.copy()
+    o.streaming()
+    for i in range(19):
+        exp = np.roll(f0[..., i], shift=(CZ[i], CY[i], CX[i]), axis=(0, 1, 2))
+        assert np.array_equal(o.f[..., i], exp)
+
+
+def test_streaming_walls_halfway_bounce_back(oracle_lib):
+    rng = np.random.default_rng(2)
+    prm = LBMParams(nx=5, ny=6, nz=4, visc=0.01, walls_y=True)
+    o = _orc(prm)
+    o.f[:] = rng.normal(size=o.f.shape)
+    f0 = o.f.copy()
+    o.streaming()
+    opp = [0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15]
+    for z in range(prm.nz):
+        for y in range(prm.ny):
+            for x in range(prm.nx):
+                for i in range(1, 19):
+                    ys = y - CY[i]
+                    if ys < 0 or ys >= prm.ny:
+                        exp = f0[z, y, x, opp[i]]
+                    else:
+                        exp = f0[(z - CZ[i]) % prm.nz, ys, (x - CX[i]) % prm.nx, i]
+                    assert o.f[z, y, x, i] == exp
+
+
+# ----------------------------------------------------------- collision -----
+def _consistent_state(o, rng):
+    o.f[:] = rng.normal(0, 1e-3, o.f.shape)
+    for n in ("fx", "fy", "fz"):
+        getattr(o, n)[:] = rng.normal(0, 1e-4, o.rho.shape)
+    o.ibnodes[:] = -1
+    o.rho[:] = o.f.sum(-1)
+    o.ux[:] = (o.f * CX).sum(-1) + o.fx / 2
+    o.uy[:] = (o.f * CY).sum(-1) + o.fy / 2
+    o.uz[:] = (o.f * CZ).sum(-1) + o.fz / 2
+
+
+def test_collision_conserves_mass_and_adds_force(oracle_lib):
+    rng = np.random.default_rng(3)
+    prm = LBMParams(nx=6, ny=5, nz=4, visc=0.02)
+    o = _orc(prm)
+    _consistent_state(o, rng)
+    rho, ux, uy, uz = o.rho.copy(), o.ux.copy(), o.uy.copy(), o.uz.copy()
+    o.collision_mrt()
+    assert np.max(np.abs(o.f.sum(-1) - rho)) < 1e-16
+    for c, u, F in ((CX, ux, o.fx), (CY, uy, o.fy), (CZ, uz, o.fz)):
+        assert np.max(np.abs((o.f * c).sum(-1) - (u + F / 2))) < 1e-16
+
+
+def test_collision_bgk_limit(oracle_lib):
+    """MRT type 2 (all rates 1/tau, equilibrium weights 3, -11/2, -1/2) is BGK:
+    f' = g - (g - feq(rho, u))/tau + Fbar/2 with g = f + Fbar/2."""
+    rng = np.random.default_rng(4)
+    prm = LBMParams(nx=5, ny=4, nz=3, visc=0.05, mrt_type=2)
+    o = _orc(prm)
+    _consistent_state(o, rng)
+    f0, rho, u = o.f.copy(), o.rho.copy(), np.stack([o.ux, o.uy, o.uz], -1)
+    F = np.stack([o.fx, o.fy, o.fz], -1)
+    o.collision_mrt()
+    tau = prm.tau
+    for idx in np.ndindex(o.rho.shape):
+        uu, FF = u[idx], F[idx]
+        cF = CX * FF[0] + CY * FF[1] + CZ * FF[2]
+        cu = CX * uu[0] + CY * uu[1] + CZ * uu[2]
+        uF = float(np.dot(uu, FF))
+        Fb = W * (3 * cF + 9 * cF * cu - 3 * uF)
+        Fb[0] = -uF
+        g = f0[idx] + Fb / 2
+        exp = g - (g - feq_np(rho[idx], uu)) / tau + Fb / 2
+        assert np.max(np.abs(o.f[idx] - exp)) < 1e-15
+
+
+def test_equilibrium_moments(oracle_lib):
+    prm = LBMParams(nx=4, ny=4, nz=4, visc=0.01)
+    o = _orc(prm)
+    rng = np.random.default_rng(5)
+    o.ux[:], o.uy[:], o.uz[:] = (rng.normal(0, 0.05, o.rho.shape) for _ in range(3))
+    o.initpop()
+    assert np.all(o.rho == 0.0)
+    u = np.stack([o.ux, o.uy, o.uz], -1)
+    assert np.max(np.abs(o.f.sum(-1))) < 1e-16
+    for a, c in enumerate((CX, CY, CZ)):
+        assert np.max(np.abs((o.f * c).sum(-1) - u[..., a])) < 4e-16 * np.max(np.abs(u))
+        for b, d in enumerate((CX, CY, CZ)):
+            flux = (o.f * c * d).sum(-1)
+            assert np.max(np.abs(flux - u[..., a] * u[..., b])) < 1e-16
+
+
+# ------------------------------------------------------------ particles -----
+def _sphere_sets(prm, centres, r):
+    """Brute force: solid nodes and boundary links with nearest images."""
+    solid, links = {}, {}
+    N = np.array([prm.nx, prm.ny, prm.nz], dtype=float)
+    for p, c in enumerate(centres):
+        for z in range(prm.nz):
+            for y in range(prm.ny):
+                for x in range(prm.nx):
+                    node = np.array([x + 0.5, y + 0.5, z + 0.5])
+                    d = node - c
+                    d -= N * np.round(d / N)
+                    if np.dot(d, d) <= r * r:
+                        solid[(z, y, x)] = p + 1
+                        continue
+                    if np.max(np.abs(d)) > r + 2:
+                        continue
+                    for q in range(1, 19):
+                        e = d + np.array([CX[q], CY[q], CZ[q]])
+                        if np.dot(e, e) <= r * r:
+                            links[(z, y, x, q)] = p + 1
+    return solid, links
+
+
+def test_links_brute_force_and_wall_point(oracle_lib):
+    rng = np.random.default_rng(6)
+    prm = LBMParams(nx=16, ny=14, nz=15, visc=0.01, npart=2, rad=3.3)
+    o = _orc(prm)
+    centres = np.array([[0.3, 7.2, 14.6], [8.9, 3.4, 6.1]])
+    o.ypglb[:] = centres
+    o.beads_links()
+    solid, links = _sphere_sets(prm, centres, prm.rad)
+    got_solid = {idx: int(o.isnodes[idx]) for idx in zip(*np.nonzero(o.ibnodes > 0))}
+    assert got_solid == solid
+    got_links = {(z, y, x, q + 1): int(o.mlink[z, y, x, q]) for z, y, x, q in zip(*np.nonzero(o.ilink > 0))}
+    assert got_links == links
+    N = np.array([prm.nx, prm.ny, prm.nz], dtype=float)
+    for (z, y, x, q), p in links.items():
+        a = o.alink[z, y, x, q - 1]
+        assert 0.0 <= a <= 1.0
+        pt = np.array([x + 0.5, y + 0.5, z + 0.5]) + a * np.array([CX[q], CY[q], CZ[q]]) - centres[p - 1]
+        pt -= N * np.round(pt / N)
+        assert abs(np.linalg.norm(pt) - prm.rad) < 1e-12
+
+
+def _bb_setup(prm, centre, f_node, wp=(0, 0, 0)):
+    o = _orc(prm)
+    o.ypglb[0] = centre
+    o.yp[0] = centre
+    o.wp[0] = wp
+    o.beads_links()
+    o.f[:] = f_node
+    return o
+
+
+def test_bounce_back_uniform_density_exerts_no_force(oracle_lib):
+    """Sphere at rest centred on a lattice-symmetric point in a uniform
+    density perturbation: every interpolation weights sum to one, so each
+    reflected value equals the incoming one and the link sums cancel."""
+    prm = LBMParams(nx=16, ny=16, nz=16, visc=0.01, npart=1, rad=3.7)
+    o = _bb_setup(prm, [8.0, 8.0, 8.0], W * 1e-3)
+    o.beads_collision()
+    assert np.max(np.abs(o.fHIp)) < 1e-17
+    assert np.max(np.abs(o.torqp)) < 1e-17
+
+
+def test_bounce_back_moving_sphere_force_closed_form(oracle_lib):
+    """Fluid at rest (f = 0), sphere translating with U: every link reflects
+    only its moving-wall term, -6 w_q (U.c_q) m(alpha) with m = 1 for
+    alpha <= 1/2 and the leading interpolation weight for alpha > 1/2
+    (1/(alpha(2 alpha + 1)), or 1/(2 alpha) when x - c is solid; Lallemand &
+    Luo 2003), so F = sum_links -6 w_q (U.c_q) m(alpha) c_q exactly."""
+    prm = LBMParams(nx=16, ny=16, nz=16, visc=0.01, npart=1, rad=3.4)
+    U = np.array([0.013, -0.007, 0.004])
+    o = _bb_setup(prm, [8.0, 7.5, 8.25], 0.0, wp=U)
+    o.beads_collision()
+    F = np.zeros(3)
+    for z, y, x, q in zip(*np.nonzero(o.ilink > 0)):
+        c = np.array([CX[q + 1], CY[q + 1], CZ[q + 1]])
+        a = o.alink[z, y, x, q]
+        back_solid = o.ibnodes[(z - c[2]) % 16, (y - c[1]) % 16, (x - c[0]) % 16] > 0
+        m = 1.0 if a <= 0.5 else (0.5 / a if back_solid else 1.0 / a / (2 * a + 1))
+        F += -6 * W[q + 1] * float(np.dot(U, c)) * m * c
+    assert np.max(np.abs(o.fHIp[0] - F)) < 1e-15
+    assert np.all(F * U < 0)   # drag opposes the motion
+
+
+def test_move_uniform_acceleration_exact(oracle_lib):
+    prm = LBMParams(nx=32, ny=32, nz=32, visc=0.01, npart=1, rad=3.0)
+    o = _orc(prm)
+    Fc, Tc = np.array([0.3, -0.2, 0.1]), np.array([0.05, 0.02, -0.04])
+    y0 = np.array([10.0, 11.0, 12.0])
+    o.yp[0] = y0
+    for n in range(1, 9):
+        o.S.istep = n          # irelease = 1: the first call starts from rest
+        o.fHIp[0] = Fc
+        o.torqp[0] = Tc
+        o.flubp[0] = 0.0
+        o.beads_move()
+        assert np.allclose(o.wp[0], n * Fc / prm.amp, rtol=0, atol=1e-16)
+        assert np.allclose(o.omgp[0], n * Tc / prm.aip, rtol=0, atol=1e-16)
+        assert np.allclose(o.yp[0], y0 + 0.5 * Fc / prm.amp * n * n, rtol=0, atol=1e-13)
+        assert np.allclose(o.thetap[0], 0.5 * Tc / prm.aip * n * n, rtol=0, atol=1e-15)
+
+
+def test_lubrication_pair_properties(oracle_lib):
+    prm = LBMParams(nx=40, ny=40, nz=40, visc=0.01, npart=2, rad=4.0)
+    o = _orc(prm)
+    dist0 = 2 * prm.rad + prm.mingap
+    e = np.array([0.6, 0.0, 0.8])
+    mags = []
+    for d in (dist0 + 0.1, dist0 - 1e-9, 9.5, 8.5, 8.0, 7.9):
+        o.yp[0] = o.ypglb[0] = [20.0, 20.0, 20.0]
+        o.ypglb[1] = o.ypglb[0] + d * e
+        o.yp[1] = o.ypglb[1]
+        o.beads_lubforce()
+        f0, f1 = o.flubp[0].copy(), o.flubp[1].copy()
+        assert np.array_equal(f0, -f1)
+        mag = np.linalg.norm(f0)
+        if mag > 0:
+            assert np.allclose(f0 / mag, -e, atol=1e-12)     # pushes particle 0 away
+        mags.append(mag)
+    assert mags[0] == 0.0
+    assert mags[1] < 1e-12 * mags[-1]          # continuous at the cut-off
+    assert mags[2] < mags[3] < mags[4] < mags[5]
+    # contact adds the second (stiffer) term: slope jumps at h = 0
+    assert (mags[5] - mags[4]) / 0.1 > (mags[4] - mags[3]) / 0.5
+
+
+def test_refill_uniform_equilibrium_closed_form(oracle_lib):
+    """Uniform equilibrium fluid: the non-equilibrium part vanishes and the
+    refilled node is feq(rho, u_w) with u_w the rigid-body velocity from the
+    new w, omega and the old centre (partlib.f90:3070-3089)."""
+    prm = LBMParams(nx=16, ny=16, nz=16, visc=0.01, npart=1, rad=3.0)
+    o = _orc(prm)
+    rho_u, u_u = 2e-3, np.array([0.01, -0.02, 0.005])
+    o.f[:] = feq_np(rho_u, u_u)
+    node = (8, 8, 4)                        # z, y, x of the uncovered node
+    o.ibnodes[:] = -1
+    o.ibnodes0[:] = -1
+    o.isnodes0[:] = -1
+    o.ibnodes0[node] = 1
+    o.isnodes0[node] = 1
+    old_c = np.array([5.6, 8.3, 8.4])
+    o.ypglbp[0] = old_c
+    o.wp[0] = [0.02, 0.01, -0.01]
+    o.wpp[0] = [0.018, 0.011, -0.009]
+    o.omgp[0] = [0.001, -0.002, 0.003]
+    o.omgpp[0] = [0.001, -0.002, 0.003]
+    o.beads_filling()
+    r = np.array([node[2] + 0.5, node[1] + 0.5, node[0] + 0.5]) - old_c
+    uw = o.wp[0] + np.cross(o.omgp[0], r)
+    assert np.max(np.abs(o.f[node] - feq_np(rho_u, uw))) < 1e-17
+
+
+def test_trilinear_exact_on_linear_field(oracle_lib):
+    prm = LBMParams(nx=20, ny=18, nz=16, visc=0.01, npart=2, rad=3.0)
+    o = _orc(prm)
+    z, y, x = np.meshgrid(np.arange(16) + 0.5, np.arange(18) + 0.5, np.arange(20) + 0.5, indexing="ij")
+    o.ux[:] = 0.1 + 0.01 * x - 0.02 * y + 0.03 * z
+    o.uy[:] = -0.2 + 0.005 * x
+    o.uz[:] = 0.3 * np.ones_like(x)
+    cs = np.array([[7.3, 8.8, 6.1], [12.45, 3.05, 9.95]])
+    o.ypglb[:] = cs
+    o.initpartvel()
+    for p, c in enumerate(cs):
+        assert abs(o.wp[p, 0] - (0.1 + 0.01 * c[0] - 0.02 * c[1] + 0.03 * c[2])) < 1e-15
+        assert abs(o.wp[p, 1] - (-0.2 + 0.005 * c[0])) < 1e-15
+        assert abs(o.wp[p, 2] - 0.3) < 1e-15
+
+
+def test_initpartpos_no_overlap_and_repeatable(oracle_lib):
+    prm = LBMParams(nx=48, ny=48, nz=48, visc=0.01, npart=20, rad=4.0)
+    pos = []
+    for _ in range(2):
+        o = _orc(prm)
+        o.S.istep00 = 0
+        o.initpartpos()
+        pos.append(o.ypglb.copy())
+    assert np.array_equal(pos[0], pos[1])
+    N = np.array([48, 48, 48.0])
+    for a in range(20):
+        for b in range(a + 1, 20):
+            d = pos[0][a] - pos[0][b]
+            d -= N * np.round(d / N)
+            assert np.linalg.norm(d) >= 2 * prm.rad + prm.mingap
+    assert np.all((pos[0] >= 0) & (pos[0] < 48))
+
+
+def test_bb_reference_order_equals_snapshot_without_contact(oracle_lib):
+    """The reference's in-place single-rank bounce-back order and the snapshot
+    order agree when no link sits on a solid node."""
+    rng = np.random.default_rng(9)
+    prm = LBMParams(nx=24, ny=22, nz=20, visc=0.01, npart=2, rad=3.5)
+    outs = []
+    for mode in (0, 1):
+        o = _orc(prm, bb_mode=mode)
+        o.ypglb[:] = [[4.1, 20.9, 2.2], [14.7, 9.3, 11.6]]   # one across the periodic corner
+        o.wp[:] = [[0.01, -0.02, 0.005], [0.0, 0.01, 0.0]]
+        o.omgp[:] = [[0.001, 0.0, -0.002], [0.0, 0.003, 0.0]]
+        o.beads_links()
+        o.f[:] = np.random.default_rng(10).normal(0, 1e-3, o.f.shape)
+        o.beads_collision()
+        assert o.S.nconflict == 0
+        outs.append((o.f.copy(), o.fHIp.copy(), o.torqp.copy()))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.allclose(outs[0][1], outs[1][1], rtol=1e-13, atol=0)
+    assert np.allclose(outs[0][2], outs[1][2], rtol=1e-13, atol=0)
+
+
+# -------------------------------------------------------------- physics -----
+def test_shear_wave_decay_rate(oracle_lib):
+    """u_x = U sin(2 pi y / L) decays as exp(-nu k^2 t) with
+    nu = (1/s9 - 1/2)/3 = (tau - 1/2)/3 (no forcing, no particles)."""
+    L = 32
+    prm = LBMParams(nx=4, ny=L, nz=4, visc=0.02, forcing=False)
+    o = _orc(prm)
+    y = np.arange(L)
+    U = 1e-3
+    o.ux[:] = U * np.sin(2 * np.pi * (y[None, :, None] + 0.0) / L)
+    o.initpop()
+    o.rhoupdat()
+    o.macrovar()
+    amps = []
+    nsteps = 400
+    for n in range(nsteps + 1):
+        if n in (100, nsteps):
+            amps.append(np.sum(o.ux[0, :, 0] * np.sin(2 * np.pi * y / L)) * 2 / L)
+        if n < nsteps:
+            o.collision_mrt()
+            o.streaming()
+            o.macrovar()
+    k = 2 * np.pi / L
+    rate = -np.log(amps[1] / amps[0]) / (nsteps - 100)
+    # lattice error of the decay rate at k = 2 pi / 32 is a few 1e-3
+    assert abs(rate - prm.visc * k * k) < 1e-2 * prm.visc * k * k
