@@ -26,13 +26,17 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {"fp32": 152, "fp64": 304}   # 19 populations read + 19 written
+# HBM bytes per k_collide dispatch, measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE on this workload (FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0, the
+# factors a plain 1 GiB device copy gives): profiles/hbm_traffic_config2_256cube_300spheres.md
+MEASURED_TRAFFIC = {("config2", "fp32"): 2.60276e9, ("config2", "fp64"): 5.15306e9}
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_gpu(case, precision, steps, warmup, field):
+def run_gpu(case, precision, steps, warmup, field, case_name):
     from multiphase_flow_codes_amd.workloads import setup_lattice
     t0 = time.time()
     lat, prm, field = setup_lattice(case, precision, field)
@@ -60,7 +64,8 @@ def run_gpu(case, precision, steps, warmup, field):
     return {
         "mlups": mlups, "ms_per_step": ms_step, "kernel_ms": avg_kern, "launches": launches,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC.get((case_name, precision)),
                      "kernel": f"k_collide<{ 'float' if precision == 'fp32' else 'double'}>",
                      "bytes_per_cell": BYTES_PER_CELL[precision]},
         "links": st["nlinks"],
@@ -97,6 +102,20 @@ def run_cpu_baseline(case, field, nsteps):
                       f"oracle (reference single-rank order), OpenMP on the fluid loops"}
 
 
+def calibrate_copy(mib):
+    """A plain device-to-device copy of known size: under rocprofv3 --pmc its
+    FETCH_SIZE / WRITE_SIZE per dispatch against mib*1024 KiB gives the scale
+    factor of those counters on this device."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.rand(n, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy calibration: 5 copies of {mib} MiB done")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,7 +125,14 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true",
+                    help="skip the FP32 path (profiling runs of the FP64 kernels only)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-sized buffer 5 times (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a "
                                    "design note (z-slab halos), not an implemented path",
@@ -114,10 +140,14 @@ def main():
         sys.exit(2)
     from multiphase_flow_codes_amd import workloads
     case = workloads.CONFIG2 if args.case == "config2" else workloads.CONFIG1
-    r32, field = run_gpu(case, "fp32", args.steps, args.warmup, None)
-    r64 = None
+    r32 = r64 = field = None
+    if not args.no_fp32:
+        r32, field = run_gpu(case, "fp32", args.steps, args.warmup, None, args.case)
     if not args.no_fp64:
-        r64, _ = run_gpu(case, "fp64", args.steps, args.warmup, field)
+        r64, field = run_gpu(case, "fp64", args.steps, args.warmup, field, args.case)
+    if r32 is None:
+        log("FP32 path skipped: no headline line printed")
+        return
     cpu = None
     if not args.no_cpu:
         cpu = run_cpu_baseline(case, field, args.cpu_steps)

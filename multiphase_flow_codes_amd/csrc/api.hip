@@ -360,11 +360,13 @@ int mfc_destroy(mfc_ctx* ctx) {
   F(c.ft.d_modes); F(c.ft.phx);
   Particles& P = c.pt;
   double* pd[] = {P.yp, P.ypglb, P.ypglbp, P.wp, P.wpp, P.omgp, P.omgpp, P.fHIp, P.flubp, P.torqp,
-                  P.forcep, P.forcepp, P.torqpp, P.dwdt, P.domgdt, P.thetap, P.map_pos[0], P.map_pos[1],
-                  P.lk_alpha, P.lk_out, P.seg_ft};
+                  P.forcep, P.forcepp, P.torqpp, P.dwdt, P.domgdt, P.thetap, P.lk_alpha, P.lk_out,
+                  P.seg_ft};
   for (double* a : pd) F(a);
   F(P.isn[0]); F(P.isn[1]); F(P.mask[0]); F(P.mask[1]);
-  F(P.lk_node); F(P.lk_dir); F(P.lk_cnt); F(P.lk_off); F(P.sol_cnt);
+  F(P.lk_node); F(P.lk_dir); F(P.lk_cnt); F(P.lk_off);
+  F(P.sol_list[0]); F(P.sol_list[1]); F(P.sol_cnt[0]); F(P.sol_cnt[1]);
+  F(P.sh_node); F(P.sh_hits); F(P.sh_cnt);
   F(P.fill_list); F(P.fill_count);
   F(c.d_status); F(c.d_red); F(c.staging);
   if (c.h_modes) hipHostFree(c.h_modes);
@@ -625,26 +627,39 @@ int mfc_prt_init(mfc_ctx* ctx, int32_t npart) {
     if (c.g.nx < need || c.g.nz < need || (!c.g.walls_y && c.g.ny < need))
       return fail(ctx, MFC_ERR_ARG, "box too small for the particle radius (need > 2*(rad+2)+5 nodes)");
     if (!(c.p.rad > 0.0)) return fail(ctx, MFC_ERR_ARG, "particle radius must be positive");
+    if ((size_t)segment_capacity(c.p.rad) * sizeof(uint32_t) > 65536)
+      return fail(ctx, MFC_ERR_ARG, "particle radius too large for the box-scan kernels (rad <= 27)");
   }
   HIPCHK(hipStreamSynchronize(c.stream));
   if (npart > P.np_max) {
     double** pd[] = {&P.yp, &P.ypglb, &P.ypglbp, &P.wp, &P.wpp, &P.omgp, &P.omgpp, &P.fHIp, &P.flubp,
-                     &P.torqp, &P.forcep, &P.forcepp, &P.torqpp, &P.dwdt, &P.domgdt, &P.thetap,
-                     &P.map_pos[0], &P.map_pos[1]};
+                     &P.torqp, &P.forcep, &P.forcepp, &P.torqpp, &P.dwdt, &P.domgdt, &P.thetap};
     for (double** a : pd) {
       if (*a) HIPCHK(hipFree(*a));
       *a = nullptr;
       HIPCHK(hipMalloc(a, sizeof(double) * 3 * npart));
     }
     P.nseg = segments_per_particle(c.p.rad);
+    P.seg_cap = segment_capacity(c.p.rad);
     const size_t nsegs = (size_t)npart * P.nseg;
-    if (P.lk_cnt) HIPCHK(hipFree(P.lk_cnt));
-    if (P.sol_cnt) HIPCHK(hipFree(P.sol_cnt));
-    if (P.lk_off) HIPCHK(hipFree(P.lk_off));
-    if (P.seg_ft) HIPCHK(hipFree(P.seg_ft));
+    const size_t nodes = nsegs * (size_t)P.seg_cap;
+    void** segbufs[] = {(void**)&P.lk_cnt, (void**)&P.lk_off, (void**)&P.seg_ft,
+                        (void**)&P.sol_list[0], (void**)&P.sol_list[1], (void**)&P.sol_cnt[0],
+                        (void**)&P.sol_cnt[1], (void**)&P.sh_node, (void**)&P.sh_hits,
+                        (void**)&P.sh_cnt};
+    for (void** a : segbufs) {
+      if (*a) HIPCHK(hipFree(*a));
+      *a = nullptr;
+    }
     HIPCHK(hipMalloc(&P.lk_cnt, sizeof(int32_t) * nsegs));
-    HIPCHK(hipMalloc(&P.sol_cnt, sizeof(int32_t) * nsegs));
     HIPCHK(hipMalloc(&P.lk_off, sizeof(int64_t) * (nsegs + 1)));
+    for (int b = 0; b < 2; ++b) {
+      HIPCHK(hipMalloc(&P.sol_list[b], sizeof(uint32_t) * nodes));
+      HIPCHK(hipMalloc(&P.sol_cnt[b], sizeof(int32_t) * nsegs));
+    }
+    HIPCHK(hipMalloc(&P.sh_node, sizeof(uint32_t) * nodes));
+    HIPCHK(hipMalloc(&P.sh_hits, sizeof(uint32_t) * nodes));
+    HIPCHK(hipMalloc(&P.sh_cnt, sizeof(int32_t) * nsegs));
     HIPCHK(hipMalloc(&P.seg_ft, sizeof(double) * 6 * nsegs));
     // every refilled node was solid: at most the solid nodes of all spheres
     const double rs = c.p.rad + 1.0;

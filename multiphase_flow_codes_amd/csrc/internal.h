@@ -53,19 +53,30 @@ struct Particles {
   // node maps, double-buffered: isn = particle id (1-based) or 0, mask = solid bit
   int32_t* isn[2] = {nullptr, nullptr};
   uint32_t* mask[2] = {nullptr, nullptr};
-  double* map_pos[2] = {nullptr, nullptr};   // centres each map was built from
   int map_np[2] = {0, 0};                    // particle count of that build (0: map empty)
   int map_cur = 0, map_old = 0;
-  // boundary links, grouped by particle
+  // Box segments: one z-slab of a particle's bounding box, the work unit of
+  // the box kernels (segment s of particle p is p * nseg + s).  seg_cap bounds
+  // the nodes of one segment, so per-segment lists never overflow.
+  int nseg = 1;
+  int seg_cap = 0;
+  // solid nodes each map build marked, per segment: what the build two steps
+  // later clears and what beads_filling scans for uncovered nodes
+  uint32_t* sol_list[2] = {nullptr, nullptr};
+  int32_t* sol_cnt[2] = {nullptr, nullptr};
+  // fluid nodes within r + 1.5 of a centre and the link directions they see
+  // (bit q set: direction q crosses the surface), per segment
+  uint32_t* sh_node = nullptr;
+  uint32_t* sh_hits = nullptr;
+  int32_t* sh_cnt = nullptr;
+  // boundary links, grouped by segment
   int64_t lk_cap = 0;
   uint32_t* lk_node = nullptr;
   uint8_t* lk_dir = nullptr;
   double* lk_alpha = nullptr;
   double* lk_out = nullptr;
-  int nseg = 1;                // box segments (z-slabs) per particle
   int32_t* lk_cnt = nullptr;   // links per segment
   int64_t* lk_off = nullptr;   // link offset per segment (np * nseg + 1)
-  int32_t* sol_cnt = nullptr;  // solid nodes per segment
   double* seg_ft = nullptr;    // force and torque per segment (6 each)
   // nodes to refill, found by beads_filling's first pass
   uint64_t* fill_list = nullptr;
@@ -123,6 +134,7 @@ hipError_t launch_force_field(Ctx& c, int slot, double* fx, double* fy, double* 
 
 // ---- launchers (particles.hip) ----
 int segments_per_particle(double rad);
+int segment_capacity(double rad);
 hipError_t launch_links(Ctx& c, int target, int overlap_check, int istep);
 hipError_t launch_bounce_back(Ctx& c);
 hipError_t launch_lubforce(Ctx& c);

@@ -132,28 +132,30 @@ __device__ bool node_geom(const BoxInfo& B, const PGeom& G, int iw, int jw, int 
   return true;
 }
 
-// Walk of one segment (z-slab s) of the box in passes of PB candidate nodes:
-// each 32-lane group takes a row (fixed y, z), lanes take consecutive x.
-// Every thread runs the same number of passes (barriers are allowed in fn);
+// Walk of one segment (z-slab s) of the box in passes of PB candidate nodes,
+// the slab flattened x-fastest so every lane of a pass has a node.  Every
+// thread runs the same number of passes (barriers are allowed in fn);
 // fn(active, iw, jw, kw) is called once per pass.
 template <typename F>
 __device__ __forceinline__ void walk_segment(const BoxInfo& B, const PGeom& G, int s, F&& fn) {
-  const int lane = threadIdx.x & 31, grp = threadIdx.x >> 5;
-  const int z0 = s * SLAB, z1 = min(B.az.cnt, z0 + SLAB);
-  const int nrows = z1 > z0 ? (z1 - z0) * B.ay.cnt : 0;
-  for (int r0 = 0; r0 < nrows; r0 += PB / 32) {
-    const int r = r0 + grp;
-    const bool rowok = r < nrows;
-    const int tz = rowok ? z0 + r / B.ay.cnt : 0;
-    const int ty = rowok ? r - (tz - z0) * B.ay.cnt : 0;
-    const int kw = wrap1(B.az.lo + tz, G.nz);
-    const int jw = G.walls_y ? B.ay.lo + ty : wrap1(B.ay.lo + ty, G.ny);
-    for (int tx0 = 0; tx0 < B.ax.cnt; tx0 += 32) {
-      const int tx = tx0 + lane;
-      const bool act = rowok && tx < B.ax.cnt;
-      const int iw = wrap1(B.ax.lo + (act ? tx : 0), G.nx);
-      fn(act, iw, jw, kw);
+  const int z0 = s * SLAB;
+  const int nzs = min(SLAB, B.az.cnt - z0);
+  const int nxy = B.ax.cnt * B.ay.cnt;
+  const int total = nzs > 0 ? nzs * nxy : 0;
+  for (int t0 = 0; t0 < total; t0 += PB) {
+    const int t = t0 + (int)threadIdx.x;
+    const bool act = t < total;
+    int iw = 0, jw = 0, kw = 0;
+    if (act) {
+      const int tz = t / nxy;
+      const int rem = t - tz * nxy;
+      const int ty = rem / B.ax.cnt;
+      const int tx = rem - ty * B.ax.cnt;
+      kw = wrap1(B.az.lo + z0 + tz, G.nz);
+      jw = G.walls_y ? B.ay.lo + ty : wrap1(B.ay.lo + ty, G.ny);
+      iw = wrap1(B.ax.lo + tx, G.nx);
     }
+    fn(act, iw, jw, kw);
   }
 }
 
@@ -218,34 +220,79 @@ __device__ __forceinline__ void mask_update(uint32_t* __restrict__ mask, bool on
   }
 }
 
-// Pass 1 of beads_links (partlib.f90:945-1043) for one segment of a
-// particle's box: count solid nodes and boundary links.  The same block first
-// clears the segment of the nodes this map buffer marked two steps earlier
-// (by id, so exactly those nodes are reset), and with `ovl` set the first
-// segment of each particle performs the overlap check of beads_redistribute
-// (partlib.f90:2631-2646) on the freshly wrapped positions.
+// Position of each flagged lane in a deterministic block-wide order (pass
+// order, then thread order) for two flags at once, with one barrier per call:
+// wave totals go through LDS slots that alternate between calls.  base_* hold
+// the running totals of earlier calls and are advanced by this one.
+__device__ __forceinline__ void block_place2(bool fa, bool fb, int (*wt)[PB / 64][2], int& parity,
+                                             int& base_a, int& base_b, int& at_a, int& at_b) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t ba = __ballot(fa), bb = __ballot(fb);
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  if (lane == 0) {
+    wt[parity][w][0] = __popcll(ba);
+    wt[parity][w][1] = __popcll(bb);
+  }
+  __syncthreads();
+  int pa = 0, pb = 0, ta = 0, tb = 0;
+#pragma unroll
+  for (int q = 0; q < PB / 64; ++q) {
+    const int ca = wt[parity][q][0], cb = wt[parity][q][1];
+    pa += (q < w) ? ca : 0;
+    pb += (q < w) ? cb : 0;
+    ta += ca;
+    tb += cb;
+  }
+  at_a = base_a + pa + __popcll(ba & below);
+  at_b = base_b + pb + __popcll(bb & below);
+  base_a += ta;
+  base_b += tb;
+  parity ^= 1;
+}
+
+// beads_links, first half of the map rebuild (partlib.f90:945-949): reset the
+// nodes the same map buffer marked two steps earlier, from that build's solid
+// lists.  A node claimed by several spheres is reset by its owner (the
+// highest id, the reference's isnodes) only.
 __global__ void __launch_bounds__(PB)
-    k_links_count(const double* __restrict__ ypglb, int np, int nseg, PGeom G,
-                  const double* __restrict__ clr_pos, int clr_np, int32_t* __restrict__ isn,
-                  uint32_t* __restrict__ mask, int32_t* __restrict__ lk_cnt,
-                  int32_t* __restrict__ sol_cnt, int ovl, int istep, int32_t* __restrict__ status) {
+    k_map_clear(int nseg, int seg_cap, const uint32_t* __restrict__ sol_list,
+                const int32_t* __restrict__ sol_cnt, int32_t* __restrict__ isn,
+                uint32_t* __restrict__ mask) {
+  const int b = blockIdx.x;
+  const int p = b / nseg;
+  const int cnt = sol_cnt[b];
+  const uint32_t* lst = sol_list + (size_t)b * seg_cap;
+  for (int e0 = 0; e0 < cnt; e0 += PB) {
+    const int e = e0 + (int)threadIdx.x;
+    const bool act = e < cnt;
+    const uint32_t n = act ? lst[e] : 0u;
+    const bool mine = act && isn[n] == p + 1;
+    if (mine) isn[n] = 0;
+    mask_update(mask, mine, n, false);
+  }
+}
+
+// beads_links (partlib.f90:934-1043) for one segment of a particle's box:
+// classify every box node (solid: mark the id map and mask, list it; fluid
+// within r + 1.5: keep it in LDS), then test the 18 directions of the kept
+// nodes, one node per lane, and store node and direction bits per segment.
+// With `ovl` set the first segment of each particle also performs the overlap
+// check of beads_redistribute (partlib.f90:2631-2646) on the freshly wrapped
+// positions.  Dynamic LDS: seg_cap node indices.
+__global__ void __launch_bounds__(PB)
+    k_links_scan(const double* __restrict__ ypglb, int np, int nseg, int seg_cap, PGeom G,
+                 int32_t* __restrict__ isn, uint32_t* __restrict__ mask,
+                 uint32_t* __restrict__ sol_list, int32_t* __restrict__ sol_cnt,
+                 uint32_t* __restrict__ sh_node, uint32_t* __restrict__ sh_hits,
+                 int32_t* __restrict__ sh_cnt, int32_t* __restrict__ lk_cnt, int ovl, int istep,
+                 int32_t* __restrict__ status) {
+  extern __shared__ uint32_t shell[];
   __shared__ BoxInfo B;
+  __shared__ int wt[2][PB / 64][2];
   __shared__ int sh[PB / 64];
   __shared__ int hit;
   const int b = blockIdx.x;
   const int p = b / nseg, s = b - p * nseg;
-  if (p < clr_np) {
-    box_info(clr_pos, p, G, B);
-    if (B.ok)
-      walk_segment(B, G, s, [&](bool act, int iw, int jw, int kw) {
-        LinkGeom L;
-        const uint32_t n = (uint32_t)iw + (uint32_t)G.nx * ((uint32_t)jw + (uint32_t)G.ny * (uint32_t)kw);
-        const bool mine = act && node_geom(B, G, iw, jw, kw, L) && L.rr0 <= G.rsolid2 && isn[n] == p + 1;
-        if (mine) isn[n] = 0;
-        mask_update(mask, mine, n, false);
-      });
-    __syncthreads();
-  }
   if (p >= np) return;
   if (ovl && s == 0) {
     if (threadIdx.x == 0) hit = 0;
@@ -270,29 +317,56 @@ __global__ void __launch_bounds__(PB)
     if (threadIdx.x == 0) {
       lk_cnt[b] = 0;
       sol_cnt[b] = 0;
+      sh_cnt[b] = 0;
       if (s == 0) atomicAdd(&status[ST_BOX_FAULT], 1);
     }
     return;
   }
-  int nl = 0, ns = 0;
+  uint32_t* sol = sol_list + (size_t)b * seg_cap;
+  int parity = 0, nsol = 0, nshell = 0;
   walk_segment(B, G, s, [&](bool act, int iw, int jw, int kw) {
     LinkGeom L;
-    if (!act || !node_geom(B, G, iw, jw, kw, L) || L.rr0 > G.rlink2) return;
-    if (L.rr0 - G.r2 <= 0.0) {
-      ++ns;
-    } else {
+    const bool in = act && node_geom(B, G, iw, jw, kw, L) && L.rr0 <= G.rlink2;
+    const bool solid = in && (L.rr0 - G.r2 <= 0.0);
+    const bool near = in && !solid;
+    const uint32_t n = (uint32_t)iw + (uint32_t)G.nx * ((uint32_t)jw + (uint32_t)G.ny * (uint32_t)kw);
+    if (solid) atomicMax(&isn[n], p + 1);   // the highest id containing the node wins
+    mask_update(mask, solid, n, true);
+    int at_s, at_n;
+    block_place2(solid, near, wt, parity, nsol, nshell, at_s, at_n);
+    if (solid && at_s < seg_cap) sol[at_s] = n;
+    if (near && at_n < seg_cap) shell[at_n] = n;
+  });
+  __syncthreads();
+  const int nsh = min(nshell, seg_cap);
+  uint32_t* shn = sh_node + (size_t)b * seg_cap;
+  uint32_t* shh = sh_hits + (size_t)b * seg_cap;
+  int nl = 0;
+  for (int e0 = 0; e0 < nsh; e0 += PB) {
+    const int e = e0 + (int)threadIdx.x;
+    if (e < nsh) {
+      const uint32_t n = shell[e];
+      const int iw = (int)(n % (uint32_t)G.nx);
+      const uint32_t rw = n / (uint32_t)G.nx;
+      const int jw = (int)(rw % (uint32_t)G.ny), kw = (int)(rw / (uint32_t)G.ny);
+      LinkGeom L;
+      node_geom(B, G, iw, jw, kw, L);
+      uint32_t hits = 0;
       for (int q = 1; q < NPOP; ++q) {
         double a1, a2, a3, a4;
-        if (link_hit(L, iw, jw, kw, q, G.r2, a1, a2, a3, a4)) ++nl;
+        if (link_hit(L, iw, jw, kw, q, G.r2, a1, a2, a3, a4)) hits |= 1u << q;
       }
+      shn[e] = n;
+      shh[e] = hits;
+      nl += __popc(hits);
     }
-  });
-  int tot, tots;
+  }
+  int tot;
   block_exclusive_scan(nl, sh, tot);
-  block_exclusive_scan(ns, sh, tots);
   if (threadIdx.x == 0) {
     lk_cnt[b] = tot;
-    sol_cnt[b] = tots;
+    sol_cnt[b] = min(nsol, seg_cap);
+    sh_cnt[b] = nsh;
   }
 }
 
@@ -337,38 +411,32 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
-// Pass 2 of beads_links: mark solid nodes (id map + mask) and write the links
-// of each segment at the segment's offset, in walk order.  The first segment
-// of a particle records the centre the map was built from.
+// Second half of beads_links: expand each segment's (node, direction bits)
+// into links at the segment's offset, in node order then direction order,
+// then the wall fraction of every link of the segment (partlib.f90:1013-1024),
+// one link per lane, with the node geometry rebuilt exactly as the scan saw it.
 __global__ void __launch_bounds__(PB)
-    k_links_write(const double* __restrict__ ypglb, int np, int nseg, PGeom G,
-                  const int64_t* __restrict__ off, int64_t cap, int32_t* __restrict__ isn,
-                  uint32_t* __restrict__ mask, double* __restrict__ mpos,
-                  uint32_t* __restrict__ lk_node, uint8_t* __restrict__ lk_dir) {
+    k_links_expand(const double* __restrict__ ypglb, int np, int nseg, int seg_cap, PGeom G,
+                   const uint32_t* __restrict__ sh_node, const uint32_t* __restrict__ sh_hits,
+                   const int32_t* __restrict__ sh_cnt, const int64_t* __restrict__ off, int64_t cap,
+                   uint32_t* __restrict__ lk_node, uint8_t* __restrict__ lk_dir,
+                   double* __restrict__ lk_alpha, int32_t* __restrict__ status) {
   __shared__ BoxInfo B;
   __shared__ int sh[PB / 64];
   const int b = blockIdx.x;
-  const int p = b / nseg, s = b - p * nseg;
+  const int p = b / nseg;
   if (p >= np) return;
   box_info(ypglb, p, G, B);
-  if (threadIdx.x == 0 && s == 0)
-    for (int c = 0; c < 3; ++c) mpos[3 * p + c] = ypglb[3 * p + c];
   if (!B.ok) return;
+  const int nsh = sh_cnt[b];
+  const uint32_t* shn = sh_node + (size_t)b * seg_cap;
+  const uint32_t* shh = sh_hits + (size_t)b * seg_cap;
   int64_t base = off[b];
-  walk_segment(B, G, s, [&](bool act, int iw, int jw, int kw) {
-    LinkGeom L;
-    const bool in = act && node_geom(B, G, iw, jw, kw, L) && L.rr0 <= G.rlink2;
-    const bool solid = in && (L.rr0 - G.r2 <= 0.0);
-    uint32_t hits = 0;
-    if (in && !solid) {
-      for (int q = 1; q < NPOP; ++q) {
-        double a1, a2, a3, a4;
-        if (link_hit(L, iw, jw, kw, q, G.r2, a1, a2, a3, a4)) hits |= 1u << q;
-      }
-    }
-    const uint32_t n = (uint32_t)iw + (uint32_t)G.nx * ((uint32_t)jw + (uint32_t)G.ny * (uint32_t)kw);
-    if (solid) isn[n] = p + 1;
-    mask_update(mask, solid, n, true);
+  for (int e0 = 0; e0 < nsh; e0 += PB) {
+    const int e = e0 + (int)threadIdx.x;
+    const bool act = e < nsh;
+    uint32_t hits = act ? shh[e] : 0u;
+    const uint32_t n = act ? shn[e] : 0u;
     int tot;
     const int excl = block_exclusive_scan(__popc(hits), sh, tot);
     int64_t w = base + excl;
@@ -382,23 +450,8 @@ __global__ void __launch_bounds__(PB)
       ++w;
     }
     base += tot;
-  });
-}
-
-// Wall fraction of every link (partlib.f90:1013-1024), one link per lane:
-// the node geometry is rebuilt exactly as the links pass saw it (same box
-// constants, same centre bases), then alpha from the quadratic.
-__global__ void __launch_bounds__(PB)
-    k_link_alpha(const double* __restrict__ ypglb, int np, int nseg, PGeom G,
-                 const int64_t* __restrict__ off, int64_t cap, const uint32_t* __restrict__ lk_node,
-                 const uint8_t* __restrict__ lk_dir, double* __restrict__ lk_alpha,
-                 int32_t* __restrict__ status) {
-  __shared__ BoxInfo B;
-  const int b = blockIdx.x;
-  const int p = b / nseg;
-  if (p >= np) return;
-  box_info(ypglb, p, G, B);
-  if (!B.ok) return;
+  }
+  __syncthreads();   // the segment's links, written above by this block, are read below
   const int64_t a = off[b], e = min(off[b + 1], cap);
   int alpha_bad = 0;
   for (int64_t l = a + threadIdx.x; l < e; l += PB) {
@@ -703,36 +756,47 @@ __global__ void __launch_bounds__(PB)
 }
 
 struct FillArgs {
-  const double *ypglbp, *wp, *wpp, *omgp, *omgpp, *mpos;
+  const double *ypglbp, *wp, *wpp, *omgp, *omgpp;
   const int32_t* isn_old;
   const uint32_t *mask_new, *mask_old;
+  const uint32_t* sol_old;    // solid lists of the previous map build
+  const int32_t* sol_cnt_old;
+  int seg_cap;
   double rad, rho0, ww0, ww1, ww2;
   uint64_t* list;       // (particle << 32) | node of every node to refill
   int32_t* count;       // list length (reset by k_scan_counts of the links pass)
   int64_t cap;
 };
 
-// beads_filling, pass 1 (partlib.f90:2846-2849): find the nodes that were
-// solid for particle p in the previous map and are fluid in the new one.
-// One workgroup scans one segment of the previous-step box.
+// beads_filling, pass 1 (partlib.f90:2846-2849): the nodes that were solid
+// for particle p in the previous map (p owns them there) and are fluid in the
+// new one, from the previous build's solid lists.  One workgroup per segment.
 __global__ void __launch_bounds__(PB)
-    k_fill_find(PGeom G, FillArgs fa, int np, int nseg, int32_t* __restrict__ status) {
-  __shared__ BoxInfo B;
+    k_fill_find(FillArgs fa, int nseg, int32_t* __restrict__ status) {
   const int b = blockIdx.x;
-  const int p = b / nseg, s = b - p * nseg;
-  if (p >= np) return;
-  box_info(fa.mpos, p, G, B);
-  if (!B.ok) return;
+  const int p = b / nseg;
+  const int cnt = fa.sol_cnt_old[b];
+  const uint32_t* lst = fa.sol_old + (size_t)b * fa.seg_cap;
+  const int lane = threadIdx.x & 63;
   int nfill = 0;
-  walk_segment(B, G, s, [&](bool act, int i, int j, int k) {
-    LinkGeom L;
-    if (!act || !node_geom(B, G, i, j, k, L) || L.rr0 > G.rsolid2) return;
-    const uint32_t n = (uint32_t)i + (uint32_t)G.nx * ((uint32_t)j + (uint32_t)G.ny * (uint32_t)k);
-    if (fa.isn_old[n] != p + 1 || bit_at(fa.mask_new, n)) return;
-    ++nfill;
-    const int slot = atomicAdd(fa.count, 1);
-    if (slot < fa.cap) fa.list[slot] = ((uint64_t)(uint32_t)p << 32) | n;
-  });
+  for (int e0 = 0; e0 < cnt; e0 += PB) {
+    const int e = e0 + (int)threadIdx.x;
+    const uint32_t n = e < cnt ? lst[e] : 0u;
+    const bool fill = e < cnt && fa.isn_old[n] == p + 1 && !bit_at(fa.mask_new, n);
+    // one atomic per wavefront for the list slots
+    const uint64_t bal = __ballot(fill);
+    if (bal == 0ull) continue;
+    const int leader = __ffsll((unsigned long long)bal) - 1;
+    int slot0 = 0;
+    if (lane == leader) slot0 = atomicAdd(fa.count, __popcll(bal));
+    slot0 = __shfl(slot0, leader, 64);
+    if (fill) {
+      const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+      const int slot = slot0 + __popcll(bal & below);
+      if (slot < fa.cap) fa.list[slot] = ((uint64_t)(uint32_t)p << 32) | n;
+      ++nfill;
+    }
+  }
   if (nfill) atomicAdd(&status[ST_REFILL], nfill);
 }
 
@@ -908,22 +972,31 @@ int segments_per_particle(double rad) {
   return (maxcnt + SLAB - 1) / SLAB;
 }
 
+int segment_capacity(double rad) {
+  const double radp2 = rad + 2.0;
+  const int maxcnt = (int)ceil(2.0 * radp2 + 1.0) + 4;
+  return SLAB * maxcnt * maxcnt;
+}
+
 hipError_t launch_links(Ctx& c, int target, int overlap_check, int istep) {
   Particles& P = c.pt;
   const PGeom G = pgeom(c);
   const int nsegs = P.np * P.nseg;
-  const int clr_np = P.map_np[target];
-  const int nblk = (P.np > clr_np ? P.np : clr_np) * P.nseg;
-  hipLaunchKernelGGL(k_links_count, dim3(nblk), dim3(PB), 0, c.stream, P.ypglb, P.np, P.nseg, G,
-                     P.map_pos[target], clr_np, P.isn[target], P.mask[target], P.lk_cnt, P.sol_cnt,
+  if (P.map_np[target] > 0)
+    hipLaunchKernelGGL(k_map_clear, dim3(P.map_np[target] * P.nseg), dim3(PB), 0, c.stream, P.nseg,
+                       P.seg_cap, P.sol_list[target], P.sol_cnt[target], P.isn[target],
+                       P.mask[target]);
+  P.map_np[target] = 0;
+  if (P.np <= 0) return hipGetLastError();
+  hipLaunchKernelGGL(k_links_scan, dim3(nsegs), dim3(PB), sizeof(uint32_t) * (size_t)P.seg_cap,
+                     c.stream, P.ypglb, P.np, P.nseg, P.seg_cap, G, P.isn[target], P.mask[target],
+                     P.sol_list[target], P.sol_cnt[target], P.sh_node, P.sh_hits, P.sh_cnt, P.lk_cnt,
                      overlap_check, istep, c.d_status);
-  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, c.stream, P.lk_cnt, P.sol_cnt, nsegs,
-                     P.lk_off, P.lk_cap, c.d_status, P.fill_count);
-  hipLaunchKernelGGL(k_links_write, dim3(nsegs), dim3(PB), 0, c.stream, P.ypglb, P.np, P.nseg, G,
-                     P.lk_off, P.lk_cap, P.isn[target], P.mask[target], P.map_pos[target],
-                     P.lk_node, P.lk_dir);
-  hipLaunchKernelGGL(k_link_alpha, dim3(nsegs), dim3(PB), 0, c.stream, P.ypglb, P.np, P.nseg, G,
-                     P.lk_off, P.lk_cap, P.lk_node, P.lk_dir, P.lk_alpha, c.d_status);
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, c.stream, P.lk_cnt, P.sol_cnt[target],
+                     nsegs, P.lk_off, P.lk_cap, c.d_status, P.fill_count);
+  hipLaunchKernelGGL(k_links_expand, dim3(nsegs), dim3(PB), 0, c.stream, P.ypglb, P.np, P.nseg,
+                     P.seg_cap, G, P.sh_node, P.sh_hits, P.sh_cnt, P.lk_off, P.lk_cap, P.lk_node,
+                     P.lk_dir, P.lk_alpha, c.d_status);
   P.map_np[target] = P.np;
   return hipGetLastError();
 }
@@ -986,7 +1059,9 @@ hipError_t launch_filling(Ctx& c) {
   if (P.map_np[old] == 0 || old == P.map_cur) return hipSuccess;
   FillArgs fa;
   fa.ypglbp = P.ypglbp; fa.wp = P.wp; fa.wpp = P.wpp; fa.omgp = P.omgp; fa.omgpp = P.omgpp;
-  fa.mpos = P.map_pos[old];
+  fa.sol_old = P.sol_list[old];
+  fa.sol_cnt_old = P.sol_cnt[old];
+  fa.seg_cap = P.seg_cap;
   fa.isn_old = P.isn[old];
   fa.mask_new = P.mask[P.map_cur];
   fa.mask_old = P.mask[old];
@@ -995,8 +1070,7 @@ hipError_t launch_filling(Ctx& c) {
   fa.count = P.fill_count;
   fa.cap = P.fill_cap;
   const int np = P.map_np[old];
-  hipLaunchKernelGGL(k_fill_find, dim3(np * P.nseg), dim3(PB), 0, c.stream, pgeom(c), fa, np, P.nseg,
-                     c.d_status);
+  hipLaunchKernelGGL(k_fill_find, dim3(np * P.nseg), dim3(PB), 0, c.stream, fa, P.nseg, c.d_status);
   const dim3 grid(1024), blk(256);
   if (c.prec == MFC_PREC_FP64) {
     double* A = (double*)c.pop[c.cur];
