@@ -28,7 +28,6 @@ struct mfc_ctx {
 
 namespace {
 
-constexpr int kModeRing = 8;
 constexpr int kTimingPool = 8192;
 
 int fail(mfc_ctx* ctx, int code, const char* what, hipError_t e = hipSuccess) {
@@ -164,25 +163,36 @@ void forcing_modes_host(Ctx& c) {
   }
 }
 
-// Upload c.amod into a fresh forcing table slot and make it current.
+// Build the forcing table of the modes in c.amod into a slot no queued
+// collide or macrovar reads, and make it current.
 int publish_modes(mfc_ctx* ctx) {
   Ctx& c = ctx->c;
   int slot = 0;
-  while (slot == c.ft.cur || slot == c.ft.mac) ++slot;
-  bool live = false;
-  for (int mc = 0; mc < 3; ++mc)
-    for (int a = 0; a < 150; ++a) live = live || c.amod[mc][a] != 0.0;
-  c.ft.live[slot] = live ? 1 : 0;
-  if (live) {
-    const int r = c.mode_slot;
-    c.mode_slot = (c.mode_slot + 1) % kModeRing;
-    HIPCHK(hipEventSynchronize(c.mode_ev[r]));
-    double* h = c.h_modes + 450 * r;
-    memcpy(h, c.amod, sizeof(double) * 450);
-    HIPCHK(hipMemcpyAsync(c.ft.d_modes, h, sizeof(double) * 450, hipMemcpyHostToDevice, c.stream));
-    HIPCHK(hipEventRecord(c.mode_ev[r], c.stream));
-    HIPCHK(launch_force_table(c, slot));
+  while (slot == c.ft.cur || slot == c.ft.mac || slot == c.ft.prev) ++slot;
+  ModeSet m;
+  int total = 0;
+  for (int kx = 0; kx < 3; ++kx) {
+    m.n[kx] = 0;
+    for (int k = 1; k <= 5; ++k)
+      for (int j = 1; j <= 5; ++j) {
+        const int re = mix(2 * kx + 1, j, k), im = re + 1;
+        bool live = false;
+        for (int mc = 0; mc < 3; ++mc) live = live || c.amod[mc][re] != 0.0 || c.amod[mc][im] != 0.0;
+        if (!live) continue;
+        if (total == kMaxModes) return fail(ctx, MFC_ERR_ARG, "FORCING: more live modes than the table kernel takes (xkf > sqrt(8))");
+        m.ky[total] = (int8_t)kyz_of(j);
+        m.kz[total] = (int8_t)kyz_of(k);
+        for (int mc = 0; mc < 3; ++mc) {
+          m.a[total][2 * mc] = c.amod[mc][re];
+          m.a[total][2 * mc + 1] = c.amod[mc][im];
+        }
+        ++m.n[kx];
+        ++total;
+      }
   }
+  c.ft.live[slot] = total > 0 ? 1 : 0;
+  if (total > 0) HIPCHK(launch_force_table(c, slot, m));
+  c.ft.prev = c.ft.cur;
   c.ft.cur = slot;
   return MFC_OK;
 }
@@ -261,11 +271,11 @@ int check_faults(mfc_ctx* ctx) {
 // beads_links into the map buffer that does not hold the previous step's map
 // (the one beads_filling still needs), clearing what that buffer marked
 // before.  With `overlap_check` the links pass also runs ovlpchk.
-int links_now(mfc_ctx* ctx, int overlap_check, int istep) {
+int links_now(mfc_ctx* ctx, int overlap_check, int istep, bool with_find = false) {
   Ctx& c = ctx->c;
   Particles& P = c.pt;
   const int target = (P.map_cur != P.map_old) ? P.map_cur : (P.map_cur ^ 1);
-  HIPCHK(launch_links(c, target, overlap_check, istep));
+  HIPCHK(launch_links(c, target, overlap_check, istep, with_find));
   P.map_cur = target;
   return MFC_OK;
 }
@@ -317,10 +327,9 @@ int mfc_create(const mfc_params* p, mfc_ctx** out) {
   CK(hipMemsetAsync(c.pop[0], 0, pb, c.stream));
   CK(hipMemsetAsync(c.pop[1], 0, pb, c.stream));
   const size_t gb = sizeof(double) * 18 * (size_t)p->ny * p->nz;
-  for (int s = 0; s < 3; ++s) CK(hipMalloc(&c.ft.g[s], gb));
+  for (int s = 0; s < kForceSlots; ++s) CK(hipMalloc(&c.ft.g[s], gb));
   c.ft.cur = 0;
   c.ft.mac = 0;
-  CK(hipMalloc(&c.ft.d_modes, sizeof(double) * 450));
   {
     std::vector<double> ph(2 * (p->nx + p->ny + p->nz));
     double* px = ph.data();
@@ -334,8 +343,6 @@ int mfc_create(const mfc_params* p, mfc_ctx** out) {
     c.ft.phz = c.ft.phy + 2 * p->ny;
     CK(hipMemcpy(c.ft.phx, ph.data(), sizeof(double) * ph.size(), hipMemcpyHostToDevice));
   }
-  CK(hipHostMalloc((void**)&c.h_modes, sizeof(double) * 450 * kModeRing, hipHostMallocDefault));
-  for (int r = 0; r < kModeRing; ++r) CK(hipEventCreateWithFlags(&c.mode_ev[r], hipEventDisableTiming));
   CK(hipMalloc(&c.d_status, sizeof(int32_t) * ST_COUNT));
   CK(hipMemsetAsync(c.d_status, 0, sizeof(int32_t) * ST_COUNT, c.stream));
   CK(hipHostMalloc((void**)&c.h_status, sizeof(int32_t) * ST_COUNT, hipHostMallocDefault));
@@ -356,8 +363,8 @@ int mfc_destroy(mfc_ctx* ctx) {
   };
   F(c.pop[0]); F(c.pop[1]);
   F(c.rho); F(c.ux); F(c.uy); F(c.uz);
-  for (int s = 0; s < 3; ++s) F(c.ft.g[s]);
-  F(c.ft.d_modes); F(c.ft.phx);
+  for (int s = 0; s < kForceSlots; ++s) F(c.ft.g[s]);
+  F(c.ft.phx);
   Particles& P = c.pt;
   double* pd[] = {P.yp, P.ypglb, P.ypglbp, P.wp, P.wpp, P.omgp, P.omgpp, P.fHIp, P.flubp, P.torqp,
                   P.forcep, P.forcepp, P.torqpp, P.dwdt, P.domgdt, P.thetap, P.lk_alpha, P.lk_out,
@@ -369,11 +376,8 @@ int mfc_destroy(mfc_ctx* ctx) {
   F(P.sh_node); F(P.sh_hits); F(P.sh_cnt);
   F(P.fill_list); F(P.fill_count);
   F(c.d_status); F(c.d_red); F(c.staging);
-  if (c.h_modes) hipHostFree(c.h_modes);
   if (c.h_status) hipHostFree(c.h_status);
   if (c.h_red) hipHostFree(c.h_red);
-  for (int r = 0; r < kModeRing; ++r)
-    if (c.mode_ev[r]) hipEventDestroy(c.mode_ev[r]);
   for (auto e : c.tpool) hipEventDestroy(e);
   c.tpool.clear();
   if (c.stream && c.own_stream) hipStreamDestroy(c.stream);
@@ -920,21 +924,29 @@ int mfc_step(mfc_ctx* ctx, int32_t istep, int32_t nsteps) {
   NEEDCTX;
   Ctx& c = ctx->c;
   int rc;
+  Particles& P = c.pt;
   for (int s = 0; s < nsteps; ++s) {
     const int it = istep + s;
     const bool on = particles_on(c) && it >= c.p.irelease;
     if ((rc = mfc_forcing(ctx))) return rc;
     if ((rc = mfc_collision_mrt(ctx))) return rc;
-    if (on && (rc = mfc_beads_collision(ctx))) return rc;
+    if (on) {
+      if (P.map_np[P.map_cur] == 0) return fail(ctx, MFC_ERR_STATE, "time step with particles before beads_links");
+      // bounce-back with beads_lubforce in the same grid, the scatter with the
+      // clearing of the map buffer this step's links will rebuild
+      HIPCHK(launch_bounce_back(c, true));
+      c.f_version++;
+    }
     if ((rc = mfc_streaming(ctx))) return rc;
     if (on) {
-      // lubforce; move with the periodic wrap of beads_redistribute fused in;
-      // links with its overlap check fused in (reported by mfc_sync)
-      if ((rc = mfc_beads_lubforce(ctx))) return rc;
+      // move with the periodic wrap of beads_redistribute fused in; links
+      // with its overlap check fused in (reported by mfc_sync)
       HIPCHK(launch_move(c, it == c.p.irelease ? 1 : 0, 1));
-      c.pt.map_old = c.pt.map_cur;
-      if ((rc = links_now(ctx, 1, it))) return rc;
-      if ((rc = mfc_beads_filling(ctx))) return rc;
+      P.map_old = P.map_cur;
+      // links with the refill search in the expansion grid, then the refill
+      if ((rc = links_now(ctx, 1, it, true))) return rc;
+      HIPCHK(launch_filling(c, false));
+      c.f_version++;
     }
     if ((rc = mfc_macrovar(ctx))) return rc;
   }

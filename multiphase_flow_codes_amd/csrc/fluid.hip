@@ -143,36 +143,56 @@ __global__ void __launch_bounds__(256)
 
 // Forcing table of one step: per (y,z) row the nine partial sums
 //   G_{c,kx}(y,z) = sum_{ky,kz} A_c(kx,ky,kz) exp(i 2 pi (ky y/ny + kz z/nz))
-// over the reference's 3 x 5 x 5 low modes (gaussian.f90:243-359).
-__global__ void k_force_table(const double* __restrict__ modes, const double* __restrict__ phy,
-                              const double* __restrict__ phz, int ny, int nz,
-                              double* __restrict__ G) {
+// over the live modes of the reference's 3 x 5 x 5 set (gaussian.f90:243-359).
+// The mode list arrives by value (kernel arguments); ky, kz are uniform per
+// iteration, so the phase selection below is a uniform branch.
+__device__ __forceinline__ void pick5(const double (&r)[5], const double (&i)[5], int d, double& a,
+                                      double& b) {
+  switch (d) {
+    case -2: a = r[0]; b = i[0]; break;
+    case -1: a = r[1]; b = i[1]; break;
+    case 0: a = r[2]; b = i[2]; break;
+    case 1: a = r[3]; b = i[3]; break;
+    default: a = r[4]; b = i[4]; break;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_force_table(ModeSet M, const double* __restrict__ phy, const double* __restrict__ phz, int ny,
+                  int nz, double* __restrict__ G) {
   const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= (uint32_t)ny * (uint32_t)nz) return;
   const int y = (int)(row % (uint32_t)ny), z = (int)(row / (uint32_t)ny);
-  for (int c = 0; c < 3; ++c) {
-    const double* a = modes + 150 * c;
-    for (int kx = 0; kx < 3; ++kx) {
-      double gr = 0.0, gi = 0.0;
-      for (int k = 1; k <= 5; ++k) {
-        const int kz = k <= 3 ? k - 1 : k - 6;
-        const int mz = wrapi((kz * z) % nz, nz);
-        const double ezr = phz[2 * mz], ezi = phz[2 * mz + 1];
-        for (int j = 1; j <= 5; ++j) {
-          const int ky = j <= 3 ? j - 1 : j - 6;
-          const int my = wrapi((ky * y) % ny, ny);
-          const double eyr = phy[2 * my], eyi = phy[2 * my + 1];
-          const int ai = 2 * kx + 6 * ((j - 1) + 5 * (k - 1));
-          const double ar = a[ai], aim = a[ai + 1];
-          const double er = eyr * ezr - eyi * ezi, ei = eyr * ezi + eyi * ezr;
-          gr += ar * er - aim * ei;
-          gi += ar * ei + aim * er;
-        }
+  double eyr[5], eyi[5], ezr[5], ezi[5];
+#pragma unroll
+  for (int d = -2; d <= 2; ++d) {
+    const int my = wrapi((d * y) % ny, ny), mz = wrapi((d * z) % nz, nz);
+    eyr[d + 2] = phy[2 * my];
+    eyi[d + 2] = phy[2 * my + 1];
+    ezr[d + 2] = phz[2 * mz];
+    ezi[d + 2] = phz[2 * mz + 1];
+  }
+  double g[18];
+#pragma unroll
+  for (int q = 0; q < 18; ++q) g[q] = 0.0;
+  int m = 0;
+#pragma unroll
+  for (int kx = 0; kx < 3; ++kx) {
+    for (int e = 0; e < M.n[kx]; ++e, ++m) {
+      double a, b, c, d;
+      pick5(eyr, eyi, M.ky[m], a, b);
+      pick5(ezr, ezi, M.kz[m], c, d);
+      const double er = a * c - b * d, ei = a * d + b * c;
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        const double ar = M.a[m][2 * cc], ai = M.a[m][2 * cc + 1];
+        g[6 * cc + 2 * kx] += ar * er - ai * ei;
+        g[6 * cc + 2 * kx + 1] += ar * ei + ai * er;
       }
-      G[18ull * row + 6 * c + 2 * kx] = gr;
-      G[18ull * row + 6 * c + 2 * kx + 1] = gi;
     }
   }
+#pragma unroll
+  for (int q = 0; q < 18; ++q) G[18ull * row + q] = g[q];
 }
 
 // macrovar (collision.f90:232-332): rho = sum f, u = sum c f + F/2, then the
@@ -350,9 +370,9 @@ static ForceArgs force_args(const Ctx& c) {
   return fa;
 }
 
-hipError_t launch_force_table(Ctx& c, int slot) {
+hipError_t launch_force_table(Ctx& c, int slot, const ModeSet& m) {
   const uint32_t rows = (uint32_t)c.g.ny * (uint32_t)c.g.nz;
-  hipLaunchKernelGGL(k_force_table, dim3((rows + 127) / 128), dim3(128), 0, c.stream, c.ft.d_modes,
+  hipLaunchKernelGGL(k_force_table, dim3((rows + 255) / 256), dim3(256), 0, c.stream, m,
                      c.ft.phy, c.ft.phz, c.g.ny, c.g.nz, c.ft.g[slot]);
   return hipGetLastError();
 }

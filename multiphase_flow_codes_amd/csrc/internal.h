@@ -31,15 +31,26 @@ enum StatusWord {
   ST_COUNT = 16
 };
 
+constexpr int kForceSlots = 4;
+// The live forcing modes of one FORCING call, grouped by kx (n[0] modes with
+// kx = 0 first, then kx = 1, then kx = 2), each in (kz, ky) order, with
+// (re, im) of the three components.  Handed to the table kernel by value.
+constexpr int kMaxModes = 52;
+struct ModeSet {
+  int n[3];
+  int8_t ky[kMaxModes], kz[kMaxModes];
+  double a[kMaxModes][6];
+};
+
 struct ForceTables {
-  double* g[3] = {nullptr, nullptr, nullptr};  // per (y,z) row: 3 comps x 3 kx x (re,im)
-  int live[3] = {0, 0, 0};                     // 1: table holds nonzero modes
+  double* g[kForceSlots] = {};                 // per (y,z) row: 3 comps x 3 kx x (re,im)
+  int live[kForceSlots] = {};                  // 1: table holds nonzero modes
   int cur = 0;                                 // modes of the last FORCING
   int mac = 1;                                 // modes captured by the last macrovar
+  int prev = 2;                                // modes of the FORCING before the last
   double* phx = nullptr;                       // (cos,sin)(2 pi m / nx)
   double* phy = nullptr;
   double* phz = nullptr;
-  double* d_modes = nullptr;                   // 3 x 150 mode coefficients
 };
 
 struct Particles {
@@ -104,9 +115,6 @@ struct Ctx {
   // host forcing state (gaussian.f90 FORCING)
   int32_t iseedf = 0, iyf = 0, ivf[32] = {0};
   double amod[3][150] = {{0}}, bmod[3][150] = {{0}};
-  double* h_modes = nullptr;        // pinned ring of mode uploads
-  hipEvent_t mode_ev[8] = {};
-  int mode_slot = 0;
   Particles pt;
   int32_t* d_status = nullptr;
   int32_t* h_status = nullptr;      // pinned
@@ -123,7 +131,7 @@ struct Ctx {
 };
 
 // ---- launchers (fluid.hip) ----
-hipError_t launch_force_table(Ctx& c, int slot);
+hipError_t launch_force_table(Ctx& c, int slot, const ModeSet& m);
 hipError_t launch_collide(Ctx& c, bool arrays);
 hipError_t launch_macro(Ctx& c, bool pulled);
 hipError_t launch_rhoupdat(Ctx& c, bool pulled, double* d_maxdiff);
@@ -135,12 +143,17 @@ hipError_t launch_force_field(Ctx& c, int slot, double* fx, double* fy, double* 
 // ---- launchers (particles.hip) ----
 int segments_per_particle(double rad);
 int segment_capacity(double rad);
-hipError_t launch_links(Ctx& c, int target, int overlap_check, int istep);
-hipError_t launch_bounce_back(Ctx& c);
+// with_find: the first pass of beads_filling (previous map vs the new one)
+// runs in the same grid as the link expansion
+hipError_t launch_links(Ctx& c, int target, int overlap_check, int istep, bool with_find = false);
+// with `fused` set (mfc_step) the bounce-back grid also computes the
+// lubrication forces and the scatter grid clears the next map target
+hipError_t launch_bounce_back(Ctx& c, bool fused = false);
 hipError_t launch_lubforce(Ctx& c);
+hipError_t launch_map_clear(Ctx& c, int buf);
 hipError_t launch_move(Ctx& c, int first, int wrap);
 hipError_t launch_redistribute(Ctx& c, int istep);
-hipError_t launch_filling(Ctx& c);
+hipError_t launch_filling(Ctx& c, bool find = true);
 hipError_t launch_initpartvel(Ctx& c);
 hipError_t launch_maps_to_host_layout(Ctx& c, int which, int32_t* ib, int32_t* is);
 hipError_t launch_links_dense(Ctx& c, int32_t* ilink, int32_t* mlink, double* alink);

@@ -254,11 +254,9 @@ __device__ __forceinline__ void block_place2(bool fa, bool fb, int (*wt)[PB / 64
 // nodes the same map buffer marked two steps earlier, from that build's solid
 // lists.  A node claimed by several spheres is reset by its owner (the
 // highest id, the reference's isnodes) only.
-__global__ void __launch_bounds__(PB)
-    k_map_clear(int nseg, int seg_cap, const uint32_t* __restrict__ sol_list,
-                const int32_t* __restrict__ sol_cnt, int32_t* __restrict__ isn,
-                uint32_t* __restrict__ mask) {
-  const int b = blockIdx.x;
+__device__ void clear_segment(int b, int nseg, int seg_cap, const uint32_t* __restrict__ sol_list,
+                              const int32_t* __restrict__ sol_cnt, int32_t* __restrict__ isn,
+                              uint32_t* __restrict__ mask) {
   const int p = b / nseg;
   const int cnt = sol_cnt[b];
   const uint32_t* lst = sol_list + (size_t)b * seg_cap;
@@ -270,6 +268,13 @@ __global__ void __launch_bounds__(PB)
     if (mine) isn[n] = 0;
     mask_update(mask, mine, n, false);
   }
+}
+
+__global__ void __launch_bounds__(PB)
+    k_map_clear(int nseg, int seg_cap, const uint32_t* __restrict__ sol_list,
+                const int32_t* __restrict__ sol_cnt, int32_t* __restrict__ isn,
+                uint32_t* __restrict__ mask) {
+  clear_segment(blockIdx.x, nseg, seg_cap, sol_list, sol_cnt, isn, mask);
 }
 
 // beads_links (partlib.f90:934-1043) for one segment of a particle's box:
@@ -411,19 +416,40 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
+struct FillArgs {
+  const double *ypglbp, *wp, *wpp, *omgp, *omgpp;
+  const int32_t* isn_old;
+  const uint32_t *mask_new, *mask_old;
+  const uint32_t* sol_old;    // solid lists of the previous map build
+  const int32_t* sol_cnt_old;
+  int seg_cap;
+  double rad, rho0, ww0, ww1, ww2;
+  uint64_t* list;       // (particle << 32) | node of every node to refill
+  int32_t* count;       // list length (reset by k_scan_counts of the links pass)
+  int64_t cap;
+};
+__device__ void fill_find_segment(int b, const FillArgs& fa, int nseg, int32_t* __restrict__ status);
+
 // Second half of beads_links: expand each segment's (node, direction bits)
-// into links at the segment's offset, in node order then direction order,
-// then the wall fraction of every link of the segment (partlib.f90:1013-1024),
-// one link per lane, with the node geometry rebuilt exactly as the scan saw it.
+// into links at the segment's offset, in (node, direction) order, then the
+// wall fraction of every link of the
+// segment (partlib.f90:1013-1024), one link per lane, with the node geometry
+// rebuilt exactly as the scan saw it.  Blocks past the link segments run the
+// first pass of beads_filling (the new map is complete at this point).
 __global__ void __launch_bounds__(PB)
     k_links_expand(const double* __restrict__ ypglb, int np, int nseg, int seg_cap, PGeom G,
                    const uint32_t* __restrict__ sh_node, const uint32_t* __restrict__ sh_hits,
                    const int32_t* __restrict__ sh_cnt, const int64_t* __restrict__ off, int64_t cap,
                    uint32_t* __restrict__ lk_node, uint8_t* __restrict__ lk_dir,
-                   double* __restrict__ lk_alpha, int32_t* __restrict__ status) {
+                   double* __restrict__ lk_alpha, int32_t* __restrict__ status, int nsegs,
+                   FillArgs fa) {
   __shared__ BoxInfo B;
   __shared__ int sh[PB / 64];
   const int b = blockIdx.x;
+  if (b >= nsegs) {
+    fill_find_segment(b - nsegs, fa, nseg, status);
+    return;
+  }
   const int p = b / nseg;
   if (p >= np) return;
   box_info(ypglb, p, G, B);
@@ -471,6 +497,59 @@ __global__ void __launch_bounds__(PB)
   if (alpha_bad) atomicAdd(&status[ST_ALPHA_FAULT], alpha_bad);
 }
 
+struct LubArgs {
+  double mingap, fpc, stf0, stf1;
+  double* flubp;
+};
+
+// beads_lubforce (partlib.f90:2211-2292): one workgroup sums the pair forces
+// of particle a; each pair is evaluated as the reference does, from the lower
+// id's side (so both partners see bitwise opposite pair forces).
+__device__ void lub_block(int a, const double* __restrict__ ypglb, int np, const PGeom& G,
+                          const LubArgs& L) {
+  __shared__ double red[3][PB / 64];
+  const double dist0 = 2.0 * G.rad + L.mingap;
+  const double pre = (dist0 + 1e-6) * (dist0 + 1e-6);
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  for (int b = threadIdx.x; b < np; b += PB) {
+    if (b == a) continue;
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    const double xc = ypglb[3 * lo], yc = ypglb[3 * lo + 1], zc = ypglb[3 * lo + 2];
+    const double xb = near_image(ypglb[3 * hi], xc, G.nx);
+    const double yb = G.walls_y ? ypglb[3 * hi + 1] : near_image(ypglb[3 * hi + 1], yc, G.ny);
+    const double zb = near_image(ypglb[3 * hi + 2], zc, G.nz);
+    const double dx = xc - xb, dy = yc - yb, dz = zc - zb;
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (!(d2 < pre)) continue;
+    const double dist = sqrt(d2);
+    const double fpd = lub_fpd(dist, G.rad, L.mingap, L.fpc, L.stf0, L.stf1);
+    const double sgn = (a == lo) ? 1.0 : -1.0;
+    fx += sgn * (fpd * dx);
+    fy += sgn * (fpd * dy);
+    fz += sgn * (fpd * dz);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double v[3] = {fx, fy, fz};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v[c] += __shfl_down(v[c], o, 64);
+    if (lane == 0) red[c][w] = v[c];
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < PB / 64; ++q) s += red[threadIdx.x][q];
+    L.flubp[3 * a + threadIdx.x] = s;
+  }
+}
+
+__global__ void __launch_bounds__(PB)
+    k_lubforce(const double* __restrict__ ypglb, int np, PGeom G, LubArgs L) {
+  if ((int)blockIdx.x < np) lub_block(blockIdx.x, ypglb, np, G, L);
+}
+
 __device__ __forceinline__ bool bit_at(const uint32_t* __restrict__ mask, uint32_t n) {
   return (mask[n >> 5] >> (n & 31u)) & 1u;
 }
@@ -488,9 +567,15 @@ __global__ void __launch_bounds__(PB)
                   const double* __restrict__ lk_alpha, double* __restrict__ lk_out,
                   const double* __restrict__ ypglb, const double* __restrict__ wp,
                   const double* __restrict__ omgp, double ww1, double ww2,
-                  double* __restrict__ seg_ft, int32_t* __restrict__ status) {
+                  double* __restrict__ seg_ft, int32_t* __restrict__ status, int nsegs, int np,
+                  LubArgs lub) {
   __shared__ double red[6][PB / 64];
   const int b = blockIdx.x;
+  if (b >= nsegs) {
+    // blocks past the link segments: beads_lubforce of one particle
+    lub_block(b - nsegs, ypglb, np, G, lub);
+    return;
+  }
   const int p = b / nseg;
   const int64_t a = off[b];
   const int64_t e = min(off[b + 1], cap);
@@ -525,16 +610,28 @@ __global__ void __launch_bounds__(PB)
     const bool out2 = g.walls_y && (y2 < 0 || y2 >= g.ny);
     const uint32_t m1 = out1 ? n : (uint32_t)wrap1(x - ix, g.nx) + (uint32_t)g.nx * ((uint32_t)wrap1(y1, g.ny) + (uint32_t)g.ny * (uint32_t)wrap1(z - iz, g.nz));
     const uint32_t m2 = out2 ? n : (uint32_t)wrap1(x - 2 * ix, g.nx) + (uint32_t)g.nx * ((uint32_t)wrap1(y2, g.ny) + (uint32_t)g.ny * (uint32_t)wrap1(z - 2 * iz, g.nz));
+    // only the values the interpolation uses are gathered: simple bounce-back
+    // needs f_q(x); the alpha <= 1/2 branch f_q at x - c and x - 2c, the
+    // alpha > 1/2 branch f_qbar at x and x - c
     const bool sol1 = out1 || bit_at(mask, m1);
-    const bool sol2 = out2 || bit_at(mask, m2);
     const int qo = opp(q);
-    const double ff1 = (double)A[(uint64_t)q * g.ncell + n];
-    const double ff2a = (double)A[(uint64_t)q * g.ncell + m1];
-    const double ff3a = (double)A[(uint64_t)q * g.ncell + m2];
-    const double ffo = (double)A[(uint64_t)qo * g.ncell + n];
-    const double ffo1 = (double)A[(uint64_t)qo * g.ncell + m1];
     const double wt = q <= 6 ? ww1 : ww2;
-    const double f9 = bounce_back_value(alpha, ff1, ff2a, ff3a, sol1, sol2, ffo, ffo1, wt, uwpro);
+    const double ff1 = (double)A[(uint64_t)q * g.ncell + n];
+    double f9;
+    if (alpha <= 0.5) {
+      bool sol2 = true;
+      double ff2a = 0.0, ff3a = 0.0;
+      if (!sol1) {
+        sol2 = out2 || bit_at(mask, m2);
+        ff2a = (double)A[(uint64_t)q * g.ncell + m1];
+        if (!sol2) ff3a = (double)A[(uint64_t)q * g.ncell + m2];
+      }
+      f9 = bounce_back_value(alpha, ff1, ff2a, ff3a, sol1, sol2, 0.0, 0.0, wt, uwpro);
+    } else {
+      const double ffo = (double)A[(uint64_t)qo * g.ncell + n];
+      const double ffo1 = sol1 ? 0.0 : (double)A[(uint64_t)qo * g.ncell + m1];
+      f9 = bounce_back_value(alpha, ff1, 0.0, 0.0, sol1, true, ffo, ffo1, wt, uwpro);
+    }
     lk_out[l] = f9;
     if (bit_at(mask, n)) ++contact;
     const double dff = ff1 + f9;
@@ -573,8 +670,16 @@ __global__ void __launch_bounds__(PB)
     k_bb_scatter(T* __restrict__ A, Grid g, int nseg, const int64_t* __restrict__ off, int64_t cap,
                  const uint32_t* __restrict__ lk_node, const uint8_t* __restrict__ lk_dir,
                  const double* __restrict__ lk_out, const double* __restrict__ seg_ft,
-                 double* __restrict__ fHIp, double* __restrict__ torqp) {
+                 double* __restrict__ fHIp, double* __restrict__ torqp, int nsegs, int seg_cap,
+                 const uint32_t* __restrict__ clr_list, const int32_t* __restrict__ clr_cnt,
+                 int32_t* __restrict__ clr_isn, uint32_t* __restrict__ clr_mask) {
   const int b = blockIdx.x;
+  if (b >= nsegs) {
+    // blocks past the link segments: reset one segment of the map buffer the
+    // coming beads_links rebuilds (it was built two steps ago)
+    clear_segment(b - nsegs, nseg, seg_cap, clr_list, clr_cnt, clr_isn, clr_mask);
+    return;
+  }
   const int p = b / nseg;
   if (b - p * nseg == 0 && threadIdx.x < 6) {
     double v = 0.0;
@@ -597,52 +702,6 @@ __global__ void __launch_bounds__(PB)
     if (g.walls_y && (yt < 0 || yt >= g.ny)) continue;
     const uint32_t t = (uint32_t)wrap1(x + cx(q), g.nx) + (uint32_t)g.nx * ((uint32_t)wrap1(yt, g.ny) + (uint32_t)g.ny * (uint32_t)wrap1(z + cz(q), g.nz));
     A[(uint64_t)opp(q) * g.ncell + t] = (T)lk_out[l];
-  }
-}
-
-// beads_lubforce (partlib.f90:2211-2292): one workgroup per particle sums its
-// pair forces; each pair is evaluated as the reference does, from the lower
-// id's side (so both partners see bitwise opposite pair forces).
-__global__ void __launch_bounds__(PB)
-    k_lubforce(const double* __restrict__ ypglb, int np, PGeom G, double mingap, double fpc,
-               double stf0, double stf1, double* __restrict__ flubp) {
-  __shared__ double red[3][PB / 64];
-  const int a = blockIdx.x;
-  if (a >= np) return;
-  const double dist0 = 2.0 * G.rad + mingap;
-  const double pre = (dist0 + 1e-6) * (dist0 + 1e-6);
-  double fx = 0.0, fy = 0.0, fz = 0.0;
-  for (int b = threadIdx.x; b < np; b += PB) {
-    if (b == a) continue;
-    const int lo = a < b ? a : b, hi = a < b ? b : a;
-    const double xc = ypglb[3 * lo], yc = ypglb[3 * lo + 1], zc = ypglb[3 * lo + 2];
-    const double xb = near_image(ypglb[3 * hi], xc, G.nx);
-    const double yb = G.walls_y ? ypglb[3 * hi + 1] : near_image(ypglb[3 * hi + 1], yc, G.ny);
-    const double zb = near_image(ypglb[3 * hi + 2], zc, G.nz);
-    const double dx = xc - xb, dy = yc - yb, dz = zc - zb;
-    const double d2 = dx * dx + dy * dy + dz * dz;
-    if (!(d2 < pre)) continue;
-    const double dist = sqrt(d2);
-    const double fpd = lub_fpd(dist, G.rad, mingap, fpc, stf0, stf1);
-    const double sgn = (a == lo) ? 1.0 : -1.0;
-    fx += sgn * (fpd * dx);
-    fy += sgn * (fpd * dy);
-    fz += sgn * (fpd * dz);
-  }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  double v[3] = {fx, fy, fz};
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v[c] += __shfl_down(v[c], o, 64);
-    if (lane == 0) red[c][w] = v[c];
-  }
-  __syncthreads();
-  if (threadIdx.x < 3) {
-    double s = 0.0;
-#pragma unroll
-    for (int q = 0; q < PB / 64; ++q) s += red[threadIdx.x][q];
-    flubp[3 * a + threadIdx.x] = s;
   }
 }
 
@@ -673,51 +732,96 @@ __device__ __forceinline__ bool wrap_particle(double* yp, double* ypglb, double*
 
 // beads_move (partlib.f90:2295-2394): second-order Newton-Euler update.  With
 // `wrap` set the periodic wrap of beads_redistribute (partlib.f90:2600-2626)
-// follows in the same thread.
+// follows in the same thread.  Every input is loaded before the first store
+// (the argument arrays are distinct, but the compiler cannot know it).
 __global__ void __launch_bounds__(256)
     k_move(MoveArgs m, int np, int first, int wrap, PGeom G, int32_t* __restrict__ status) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   const double dt = 1.0;
   const double bf[3] = {m.bfx, m.bfy, m.bfz};
+  double yp[3], wp[3], omgp[3], fhi[3], flub[3], torq[3], fpp[3], tpp[3], dwdt[3], domgdt[3], th[3];
+  double yg[3];
+#pragma unroll
   for (int c = 0; c < 3; ++c) {
     const int q = 3 * p + c;
-    if (first) m.thetap[q] = 0.0;
-    m.wpp[q] = m.wp[q];
-    m.omgpp[q] = m.omgp[q];
+    yp[c] = m.yp[q];
+    yg[c] = m.ypglb[q];
+    wp[c] = m.wp[q];
+    omgp[c] = m.omgp[q];
+    fhi[c] = m.fHIp[q];
+    flub[c] = m.flubp[q];
+    torq[c] = m.torqp[q];
+    fpp[c] = m.forcepp[q];
+    tpp[c] = m.torqpp[q];
+    dwdt[c] = m.dwdt[q];
+    domgdt[c] = m.domgdt[q];
+    th[c] = first ? 0.0 : m.thetap[q];
   }
-  double dwn[3], don[3];
+  double wpp[3], omgpp[3], fp[3], dwn[3], don[3];
+#pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const int q = 3 * p + c;
-    double F = m.fHIp[q] + m.flubp[q];
+    wpp[c] = wp[c];
+    omgpp[c] = omgp[c];
+    double F = fhi[c] + flub[c];
     if (c == 1) F = F - m.volp * m.rhog;
     if (bf[c] != 0.0) F = F + m.volp * bf[c];
-    m.forcep[q] = F;
-    if (first) m.forcepp[q] = F;
-    dwn[c] = 0.5 * (F + m.forcepp[q]) / m.amp;
-    if (first) m.dwdt[q] = dwn[c];
-    m.wp[q] = m.wpp[q] + 0.5 * dt * (m.dwdt[q] + dwn[c]);
+    fp[c] = F;
+    if (first) fpp[c] = F;
+    dwn[c] = 0.5 * (F + fpp[c]) / m.amp;
+    if (first) dwdt[c] = dwn[c];
+    wp[c] = wpp[c] + 0.5 * dt * (dwdt[c] + dwn[c]);
   }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) yp[c] = yp[c] + 0.5 * dt * (wp[c] + wpp[c]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (first) tpp[c] = torq[c];
+    don[c] = 0.5 * (torq[c] + tpp[c]) / m.aip;
+    if (first) domgdt[c] = don[c];
+    omgp[c] = omgpp[c] + 0.5 * dt * (domgdt[c] + don[c]);
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) th[c] = th[c] + 0.5 * dt * (omgp[c] + omgpp[c]);
+  bool bad = false;
+  double ygp[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    ygp[c] = yg[c];
+    if (wrap) {
+      // beads_redistribute: wrap into the periodic box (not across a wall),
+      // ypglbp <- ypglb, ypglb <- yp
+      const int n = c == 0 ? G.nx : (c == 1 ? G.ny : G.nz);
+      const double v = yp[c], N = (double)n;
+      if (!(G.walls_y && c == 1)) {
+        if (v >= N) yp[c] = yp[c] - N;
+        if (v < 0.0) yp[c] = yp[c] + N;
+      }
+      if (!sane_coord(yp[c], n)) bad = true;
+      ygp[c] = yg[c];
+      yg[c] = yp[c];
+    }
+  }
+#pragma unroll
   for (int c = 0; c < 3; ++c) {
     const int q = 3 * p + c;
-    m.yp[q] = m.yp[q] + 0.5 * dt * (m.wp[q] + m.wpp[q]);
-  }
-  for (int c = 0; c < 3; ++c) {
-    const int q = 3 * p + c;
-    if (first) m.torqpp[q] = m.torqp[q];
-    don[c] = 0.5 * (m.torqp[q] + m.torqpp[q]) / m.aip;
-    if (first) m.domgdt[q] = don[c];
-    m.omgp[q] = m.omgpp[q] + 0.5 * dt * (m.domgdt[q] + don[c]);
-  }
-  for (int c = 0; c < 3; ++c) {
-    const int q = 3 * p + c;
-    m.thetap[q] = m.thetap[q] + 0.5 * dt * (m.omgp[q] + m.omgpp[q]);
+    m.wpp[q] = wpp[c];
+    m.omgpp[q] = omgpp[c];
+    m.forcep[q] = fp[c];
+    m.wp[q] = wp[c];
+    m.yp[q] = yp[c];
+    m.omgp[q] = omgp[c];
+    m.thetap[q] = th[c];
     m.dwdt[q] = dwn[c];
     m.domgdt[q] = don[c];
-    m.forcepp[q] = m.forcep[q];
-    m.torqpp[q] = m.torqp[q];
+    m.forcepp[q] = fp[c];
+    m.torqpp[q] = torq[c];
+    if (wrap) {
+      m.ypglbp[q] = ygp[c];
+      m.ypglb[q] = yg[c];
+    }
   }
-  if (wrap && wrap_particle(m.yp, m.ypglb, m.ypglbp, p, G)) atomicAdd(&status[ST_NONFINITE], 1);
+  if (bad) atomicAdd(&status[ST_NONFINITE], 1);
 }
 
 // beads_redistribute, one-rank content (partlib.f90:2600-2626): periodic wrap,
@@ -755,25 +859,11 @@ __global__ void __launch_bounds__(PB)
   }
 }
 
-struct FillArgs {
-  const double *ypglbp, *wp, *wpp, *omgp, *omgpp;
-  const int32_t* isn_old;
-  const uint32_t *mask_new, *mask_old;
-  const uint32_t* sol_old;    // solid lists of the previous map build
-  const int32_t* sol_cnt_old;
-  int seg_cap;
-  double rad, rho0, ww0, ww1, ww2;
-  uint64_t* list;       // (particle << 32) | node of every node to refill
-  int32_t* count;       // list length (reset by k_scan_counts of the links pass)
-  int64_t cap;
-};
 
 // beads_filling, pass 1 (partlib.f90:2846-2849): the nodes that were solid
 // for particle p in the previous map (p owns them there) and are fluid in the
 // new one, from the previous build's solid lists.  One workgroup per segment.
-__global__ void __launch_bounds__(PB)
-    k_fill_find(FillArgs fa, int nseg, int32_t* __restrict__ status) {
-  const int b = blockIdx.x;
+__device__ void fill_find_segment(int b, const FillArgs& fa, int nseg, int32_t* __restrict__ status) {
   const int p = b / nseg;
   const int cnt = fa.sol_cnt_old[b];
   const uint32_t* lst = fa.sol_old + (size_t)b * fa.seg_cap;
@@ -798,6 +888,11 @@ __global__ void __launch_bounds__(PB)
     }
   }
   if (nfill) atomicAdd(&status[ST_REFILL], nfill);
+}
+
+__global__ void __launch_bounds__(PB)
+    k_fill_find(FillArgs fa, int nseg, int32_t* __restrict__ status) {
+  fill_find_segment(blockIdx.x, fa, nseg, status);
 }
 
 // beads_filling, pass 2 (partlib.f90:2851-3089): one wavefront per listed
@@ -978,54 +1073,98 @@ int segment_capacity(double rad) {
   return SLAB * maxcnt * maxcnt;
 }
 
-hipError_t launch_links(Ctx& c, int target, int overlap_check, int istep) {
+// Reset map buffer `buf` to empty (no-op if it is empty already).
+hipError_t launch_map_clear(Ctx& c, int buf) {
+  Particles& P = c.pt;
+  if (P.map_np[buf] > 0)
+    hipLaunchKernelGGL(k_map_clear, dim3(P.map_np[buf] * P.nseg), dim3(PB), 0, c.stream, P.nseg, P.seg_cap,
+                       P.sol_list[buf], P.sol_cnt[buf], P.isn[buf], P.mask[buf]);
+  P.map_np[buf] = 0;
+  return hipGetLastError();
+}
+
+// Arguments of beads_filling for the maps `old` (previous) and `neu` (new);
+// false if there is nothing to refill.
+static bool make_fill_args(Ctx& c, int old, int neu, FillArgs& fa) {
+  Particles& P = c.pt;
+  if (P.map_np[old] == 0 || old == neu) return false;
+  fa.ypglbp = P.ypglbp; fa.wp = P.wp; fa.wpp = P.wpp; fa.omgp = P.omgp; fa.omgpp = P.omgpp;
+  fa.sol_old = P.sol_list[old];
+  fa.sol_cnt_old = P.sol_cnt[old];
+  fa.seg_cap = P.seg_cap;
+  fa.isn_old = P.isn[old];
+  fa.mask_new = P.mask[neu];
+  fa.mask_old = P.mask[old];
+  fa.rad = c.p.rad; fa.rho0 = c.p.rho0; fa.ww0 = c.p.ww0; fa.ww1 = c.p.ww1; fa.ww2 = c.p.ww2;
+  fa.list = P.fill_list;
+  fa.count = P.fill_count;
+  fa.cap = P.fill_cap;
+  return true;
+}
+
+hipError_t launch_links(Ctx& c, int target, int overlap_check, int istep, bool with_find) {
   Particles& P = c.pt;
   const PGeom G = pgeom(c);
   const int nsegs = P.np * P.nseg;
-  if (P.map_np[target] > 0)
-    hipLaunchKernelGGL(k_map_clear, dim3(P.map_np[target] * P.nseg), dim3(PB), 0, c.stream, P.nseg,
-                       P.seg_cap, P.sol_list[target], P.sol_cnt[target], P.isn[target],
-                       P.mask[target]);
-  P.map_np[target] = 0;
+  hipError_t e = launch_map_clear(c, target);
+  if (e != hipSuccess) return e;
   if (P.np <= 0) return hipGetLastError();
+  FillArgs fa{};
+  const int nfind = (with_find && make_fill_args(c, P.map_old, target, fa)) ? P.map_np[P.map_old] * P.nseg : 0;
   hipLaunchKernelGGL(k_links_scan, dim3(nsegs), dim3(PB), sizeof(uint32_t) * (size_t)P.seg_cap,
                      c.stream, P.ypglb, P.np, P.nseg, P.seg_cap, G, P.isn[target], P.mask[target],
                      P.sol_list[target], P.sol_cnt[target], P.sh_node, P.sh_hits, P.sh_cnt, P.lk_cnt,
                      overlap_check, istep, c.d_status);
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, c.stream, P.lk_cnt, P.sol_cnt[target],
                      nsegs, P.lk_off, P.lk_cap, c.d_status, P.fill_count);
-  hipLaunchKernelGGL(k_links_expand, dim3(nsegs), dim3(PB), 0, c.stream, P.ypglb, P.np, P.nseg,
-                     P.seg_cap, G, P.sh_node, P.sh_hits, P.sh_cnt, P.lk_off, P.lk_cap, P.lk_node,
-                     P.lk_dir, P.lk_alpha, c.d_status);
+  hipLaunchKernelGGL(k_links_expand, dim3(nsegs + nfind), dim3(PB), 0, c.stream, P.ypglb, P.np,
+                     P.nseg, P.seg_cap, G, P.sh_node, P.sh_hits, P.sh_cnt, P.lk_off, P.lk_cap,
+                     P.lk_node, P.lk_dir, P.lk_alpha, c.d_status, nsegs, fa);
   P.map_np[target] = P.np;
   return hipGetLastError();
 }
 
+static LubArgs lub_args(Ctx& c) {
+  LubArgs L;
+  L.mingap = c.p.mingap;
+  L.fpc = c.p.volp * c.p.rho0 * 9.8 / c.p.gscale;
+  L.stf0 = c.p.stf0;
+  L.stf1 = c.p.stf1;
+  L.flubp = c.pt.flubp;
+  return L;
+}
+
 template <typename T>
-static void bb_dispatch(Ctx& c) {
+static void bb_dispatch(Ctx& c, bool fused) {
   Particles& P = c.pt;
   T* A = (T*)c.pop[c.cur];
   const int nsegs = P.np * P.nseg;
-  hipLaunchKernelGGL((k_bounce_back<T>), dim3(nsegs), dim3(PB), 0, c.stream, A, c.g, pgeom(c),
+  const int nlub = fused ? P.np : 0;
+  hipLaunchKernelGGL((k_bounce_back<T>), dim3(nsegs + nlub), dim3(PB), 0, c.stream, A, c.g, pgeom(c),
                      P.mask[P.map_cur], P.nseg, P.lk_off, P.lk_cap, P.lk_node, P.lk_dir, P.lk_alpha,
-                     P.lk_out, P.ypglb, P.wp, P.omgp, c.p.ww1, c.p.ww2, P.seg_ft, c.d_status);
-  hipLaunchKernelGGL((k_bb_scatter<T>), dim3(nsegs), dim3(PB), 0, c.stream, A, c.g, P.nseg, P.lk_off,
-                     P.lk_cap, P.lk_node, P.lk_dir, P.lk_out, P.seg_ft, P.fHIp, P.torqp);
+                     P.lk_out, P.ypglb, P.wp, P.omgp, c.p.ww1, c.p.ww2, P.seg_ft, c.d_status, nsegs,
+                     P.np, lub_args(c));
+  // the map buffer the next beads_links rebuilds (the previous map stays)
+  const int t = P.map_cur ^ 1;
+  const int nclr = (fused && P.map_np[t] > 0) ? P.map_np[t] * P.nseg : 0;
+  hipLaunchKernelGGL((k_bb_scatter<T>), dim3(nsegs + nclr), dim3(PB), 0, c.stream, A, c.g, P.nseg,
+                     P.lk_off, P.lk_cap, P.lk_node, P.lk_dir, P.lk_out, P.seg_ft, P.fHIp, P.torqp,
+                     nsegs, P.seg_cap, P.sol_list[t], P.sol_cnt[t], P.isn[t], P.mask[t]);
+  if (nclr) P.map_np[t] = 0;
 }
 
-hipError_t launch_bounce_back(Ctx& c) {
+hipError_t launch_bounce_back(Ctx& c, bool fused) {
   if (c.prec == MFC_PREC_FP64)
-    bb_dispatch<double>(c);
+    bb_dispatch<double>(c, fused);
   else
-    bb_dispatch<float>(c);
+    bb_dispatch<float>(c, fused);
   return hipGetLastError();
 }
 
 hipError_t launch_lubforce(Ctx& c) {
   Particles& P = c.pt;
-  const double fpc = c.p.volp * c.p.rho0 * 9.8 / c.p.gscale;
   hipLaunchKernelGGL(k_lubforce, dim3(P.np), dim3(PB), 0, c.stream, P.ypglb, P.np, pgeom(c),
-                     c.p.mingap, fpc, c.p.stf0, c.p.stf1, P.flubp);
+                     lub_args(c));
   return hipGetLastError();
 }
 
@@ -1053,24 +1192,13 @@ hipError_t launch_redistribute(Ctx& c, int istep) {
   return hipGetLastError();
 }
 
-hipError_t launch_filling(Ctx& c) {
+hipError_t launch_filling(Ctx& c, bool find) {
   Particles& P = c.pt;
-  const int old = P.map_old;
-  if (P.map_np[old] == 0 || old == P.map_cur) return hipSuccess;
   FillArgs fa;
-  fa.ypglbp = P.ypglbp; fa.wp = P.wp; fa.wpp = P.wpp; fa.omgp = P.omgp; fa.omgpp = P.omgpp;
-  fa.sol_old = P.sol_list[old];
-  fa.sol_cnt_old = P.sol_cnt[old];
-  fa.seg_cap = P.seg_cap;
-  fa.isn_old = P.isn[old];
-  fa.mask_new = P.mask[P.map_cur];
-  fa.mask_old = P.mask[old];
-  fa.rad = c.p.rad; fa.rho0 = c.p.rho0; fa.ww0 = c.p.ww0; fa.ww1 = c.p.ww1; fa.ww2 = c.p.ww2;
-  fa.list = P.fill_list;
-  fa.count = P.fill_count;
-  fa.cap = P.fill_cap;
-  const int np = P.map_np[old];
-  hipLaunchKernelGGL(k_fill_find, dim3(np * P.nseg), dim3(PB), 0, c.stream, fa, P.nseg, c.d_status);
+  if (!make_fill_args(c, P.map_old, P.map_cur, fa)) return hipSuccess;
+  if (find)
+    hipLaunchKernelGGL(k_fill_find, dim3(P.map_np[P.map_old] * P.nseg), dim3(PB), 0, c.stream, fa,
+                       P.nseg, c.d_status);
   const dim3 grid(1024), blk(256);
   if (c.prec == MFC_PREC_FP64) {
     double* A = (double*)c.pop[c.cur];
