@@ -25,7 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL = {"fp32": 152, "fp64": 304}   # 19 populations read + 19 written
+BYTES_PER_CELL = {"fp32": 152, "fp32x": 152, "fp64": 304}   # 19 populations read + 19 written
 # HBM bytes per k_collide dispatch, measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on this workload (FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0, the
 # factors a plain 1 GiB device copy gives): profiles/hbm_traffic_config2_256cube_300spheres.md
@@ -66,10 +66,49 @@ def run_gpu(case, precision, steps, warmup, field, case_name):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": MEASURED_TRAFFIC.get((case_name, precision)),
-                     "kernel": f"k_collide<{ 'float' if precision == 'fp32' else 'double'}>",
+                     "kernel": {"fp32": "k_collide<float,float>", "fp32x": "k_collide<float,double>",
+                                "fp64": "k_collide<double,double>"}[precision],
                      "bytes_per_cell": BYTES_PER_CELL[precision]},
         "links": st["nlinks"],
     }, field
+
+
+def run_accuracy(case, field, nsteps, modes=("fp32",)):
+    """FP32-storage paths against the plain FP64 path (the drop-in, which
+    matches the FP64 oracle to <= 1e-12 per step) after nsteps from the same
+    state: velocity relative L2, the statistc energy spectrum and the
+    per-particle hydrodynamic force."""
+    import numpy as np
+    from multiphase_flow_codes_amd.workloads import setup_lattice
+    out = {}
+    for prec in ("fp64",) + tuple(modes):
+        lat, prm, _ = setup_lattice(case, prec, field)
+        lat.step(1, nsteps)
+        lat.sync()
+        _, ux, uy, uz = lat.download_macro()
+        nek = case.n // 2 - 1
+        out[prec] = {"u": np.stack([ux, uy, uz]), "ek": lat.diag_spectrum(nek),
+                     "f": lat.prt_get("fHIp")[: case.npart], "sums": lat.diag_sums()}
+        lat.close()
+        log(f"accuracy: {prec} {nsteps} steps done")
+    b = out["fp64"]
+    res = {}
+    for mode in modes:
+        a = out[mode]
+        du = float(np.linalg.norm(a["u"] - b["u"]) / np.linalg.norm(b["u"]))
+        ek_rel = np.abs(a["ek"] - b["ek"]) / np.maximum(b["ek"], 1e-300)
+        fn = np.linalg.norm(b["f"], axis=1)
+        dfn = np.linalg.norm(a["f"] - b["f"], axis=1)
+        res[mode] = {
+            "steps": nsteps, "reference": "plain FP64 GPU path",
+            "velocity_rel_l2": du,
+            "spectrum_max_rel": float(ek_rel.max()),
+            "spectrum_energy_weighted_rel": float(np.abs(a["ek"] - b["ek"]).sum() / b["ek"].sum()),
+            "drag_rel_l2": float(np.linalg.norm(a["f"] - b["f"]) / np.linalg.norm(b["f"])),
+            "drag_max_rel": float((dfn / np.maximum(fn, 1e-300)).max()),
+            "kinetic_energy_rel": float(abs(a["sums"]["u2"] - b["sums"]["u2"]) / b["sums"]["u2"])}
+        log(f"accuracy {mode}: {res[mode]}")
+    return res
 
 
 def run_cpu_baseline(case, field, nsteps):
@@ -124,7 +163,11 @@ def main():
     ap.add_argument("--case", default="config2", choices=["config1", "config2"])
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="steps of the FP32-vs-FP64 accuracy comparison (0: skip)")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-fp32x", action="store_true",
+                    help="skip the FP32-storage / FP64-collision variant")
     ap.add_argument("--no-fp32", action="store_true",
                     help="skip the FP32 path (profiling runs of the FP64 kernels only)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -145,9 +188,16 @@ def main():
         r32, field = run_gpu(case, "fp32", args.steps, args.warmup, None, args.case)
     if not args.no_fp64:
         r64, field = run_gpu(case, "fp64", args.steps, args.warmup, field, args.case)
+    r32x = None
+    if not args.no_fp32x:
+        r32x, field = run_gpu(case, "fp32x", args.steps, args.warmup, field, args.case)
     if r32 is None:
         log("FP32 path skipped: no headline line printed")
         return
+    acc = None
+    if args.accuracy_steps > 0 and not args.no_fp64:
+        acc = run_accuracy(case, field, args.accuracy_steps,
+                           ("fp32",) if args.no_fp32x else ("fp32", "fp32x"))
     cpu = None
     if not args.no_cpu:
         cpu = run_cpu_baseline(case, field, args.cpu_steps)
@@ -174,6 +224,16 @@ def main():
         out["fp64"] = {"value": round(r64["mlups"], 1), "unit": "MLUPS",
                        "ms_per_step": round(r64["ms_per_step"], 4), "roofline": r64["roofline"]}
         out["fp32_over_fp64"] = round(r32["mlups"] / r64["mlups"], 3)
+    if r32x is not None:
+        out["fp32_storage_fp64_collision"] = {
+            "value": round(r32x["mlups"], 1), "unit": "MLUPS", "ms_per_step": round(r32x["ms_per_step"], 4),
+            "roofline": r32x["roofline"]}
+        if r64 is not None:
+            out["fp32_storage_fp64_collision"]["over_fp64"] = round(r32x["mlups"] / r64["mlups"], 3)
+    if acc is not None:
+        out["accuracy_fp32_vs_fp64"] = acc["fp32"]
+        if "fp32x" in acc:
+            out["fp32_storage_fp64_collision"]["accuracy_vs_fp64"] = acc["fp32x"]
     print(json.dumps(out), flush=True)
 
 

@@ -41,8 +41,9 @@ extern "C" {
 #define MFC_ERR_NONFINITE 6  /* particle state not finite or outside the box */
 #define MFC_ERR_NODEVICE 7   /* no HIP device */
 
-#define MFC_PREC_FP64 0
-#define MFC_PREC_FP32 1
+#define MFC_PREC_FP64 0          /* populations and arithmetic in FP64 (the drop-in) */
+#define MFC_PREC_FP32 1          /* populations stored and collided in FP32 */
+#define MFC_PREC_FP32_F64MATH 2  /* populations stored in FP32, collided in FP64 */
 
 /* Run constants: what the reference holds in module var_inc after `para`
  * (BOX/para.f90:1-288, BOX/var_inc.f90:14-29). */
@@ -147,6 +148,19 @@ int mfc_get_links(mfc_ctx* ctx, int32_t* ilink, int32_t* mlink, double* alink);
  * number istep (the release at istep==irelease is the caller's, as in main). */
 int mfc_step(mfc_ctx* ctx, int32_t istep, int32_t nsteps);
 /* Accumulated device time of the collide-stream kernel while timing is on. */
+/* ---- FP64 diagnostics on the device ---------------------------------------- */
+/* Sums over all nodes of the fields macrovar defines (BOX/collision.f90:232-332,
+ * as statistc and rmsstat read them, BOX/saveload.f90:1111-1300):
+ * out[0] = sum rho, out[1..3] = sum ux, uy, uz, out[4] = sum |u|^2,
+ * out[5] = max |u|^2.  Fixed reduction order: bitwise repeatable. */
+int mfc_diag_sums(mfc_ctx* ctx, double* out6);
+/* Kinetic-energy spectrum of statistc (BOX/saveload.f90:1273-1300): u
+ * transformed with forward sign and 1/n per axis (BOX/mpifft.f90:52-108);
+ * ek[ik-1] = sum over modes 0 <= kx < nx/2 (weight 1/2 at kx = 0), all ky, kz,
+ * with int(|k| + 0.5) = ik, for ik = 1..nek.  Uses a complex FP64 scratch
+ * field of 16 bytes per node, allocated on first use. */
+int mfc_diag_spectrum(mfc_ctx* ctx, int32_t nek, double* ek);
+
 int mfc_set_timing(mfc_ctx* ctx, int32_t on);
 int mfc_get_timing(mfc_ctx* ctx, double* total_ms, int64_t* launches);
 

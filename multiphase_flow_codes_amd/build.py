@@ -25,6 +25,7 @@ SOURCES = {
     "api.hip": [],
     "fluid.hip": [],
     "particles.hip": ["-ffp-contract=off"],
+    "diag.hip": [],
 }
 
 

@@ -293,7 +293,9 @@ int mfc_create(const mfc_params* p, mfc_ctx** out) {
   if (p->nx < 4 || p->ny < 4 || p->nz < 4) return MFC_ERR_ARG;
   const uint64_t ncell = (uint64_t)p->nx * p->ny * p->nz;
   if (ncell >= (1ull << 31)) return MFC_ERR_ARG;
-  if (p->precision != MFC_PREC_FP64 && p->precision != MFC_PREC_FP32) return MFC_ERR_ARG;
+  if (p->precision != MFC_PREC_FP64 && p->precision != MFC_PREC_FP32 &&
+      p->precision != MFC_PREC_FP32_F64MATH)
+    return MFC_ERR_ARG;
   mfc_ctx* ctx = new (std::nothrow) mfc_ctx();
   if (!ctx) return MFC_ERR_ARG;
   Ctx& c = ctx->c;
@@ -376,6 +378,8 @@ int mfc_destroy(mfc_ctx* ctx) {
   F(P.sh_node); F(P.sh_hits); F(P.sh_cnt);
   F(P.fill_list); F(P.fill_count);
   F(c.d_status); F(c.d_red); F(c.staging);
+  F(c.diag_part); F(c.diag_out); F(c.diag_Z);
+  for (int a = 0; a < 3; ++a) F(c.diag_tw[a]);
   if (c.h_status) hipHostFree(c.h_status);
   if (c.h_red) hipHostFree(c.h_red);
   for (auto e : c.tpool) hipEventDestroy(e);
@@ -950,6 +954,68 @@ int mfc_step(mfc_ctx* ctx, int32_t istep, int32_t nsteps) {
     }
     if ((rc = mfc_macrovar(ctx))) return rc;
   }
+  return MFC_OK;
+}
+
+// ---- diagnostics -------------------------------------------------------------------
+static int ensure_diag(mfc_ctx* ctx, int nek) {
+  Ctx& c = ctx->c;
+  const int nb = diag_blocks();
+  const int need = nek > 6 ? nek : 6;
+  if (c.diag_part && need > c.diag_nek) {
+    HIPCHK(hipStreamSynchronize(c.stream));
+    HIPCHK(hipFree(c.diag_part));
+    HIPCHK(hipFree(c.diag_out));
+    c.diag_part = nullptr;
+    c.diag_out = nullptr;
+  }
+  if (!c.diag_part) {
+    HIPCHK(hipMalloc(&c.diag_part, sizeof(double) * (size_t)nb * need));
+    HIPCHK(hipMalloc(&c.diag_out, sizeof(double) * need));
+    c.diag_nek = need;
+  }
+  return MFC_OK;
+}
+
+int mfc_diag_sums(mfc_ctx* ctx, double* out6) {
+  NEEDCTX;
+  if (!out6) return MFC_ERR_ARG;
+  Ctx& c = ctx->c;
+  int rc = need_arrays(ctx);
+  if (rc) return rc;
+  if ((rc = ensure_diag(ctx, 6))) return rc;
+  HIPCHK(launch_field_sums(c, c.diag_part, c.diag_out));
+  HIPCHK(hipMemcpyAsync(out6, c.diag_out, sizeof(double) * 6, hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return MFC_OK;
+}
+
+int mfc_diag_spectrum(mfc_ctx* ctx, int32_t nek, double* ek) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  if (!ek || nek < 1 || nek > 4096) return MFC_ERR_ARG;
+  const int ns[3] = {c.g.nx, c.g.ny, c.g.nz};
+  for (int a = 0; a < 3; ++a)
+    if (ns[a] > 2048) return fail(ctx, MFC_ERR_ARG, "diag_spectrum: an axis longer than 2048 nodes");
+  int rc = need_arrays(ctx);
+  if (rc) return rc;
+  if ((rc = ensure_diag(ctx, nek))) return rc;
+  if (!c.diag_Z) {
+    HIPCHK(hipMalloc(&c.diag_Z, sizeof(double2) * c.g.ncell));
+    for (int a = 0; a < 3; ++a) {
+      std::vector<double2> tw(ns[a]);
+      for (int j = 0; j < ns[a]; ++j) {
+        const double ang = -c.p.pi2 * (double)j / (double)ns[a];
+        tw[j] = make_double2(cos(ang), sin(ang));
+      }
+      HIPCHK(hipMalloc(&c.diag_tw[a], sizeof(double2) * ns[a]));
+      HIPCHK(hipMemcpy(c.diag_tw[a], tw.data(), sizeof(double2) * ns[a], hipMemcpyHostToDevice));
+    }
+  }
+  const double2* tw[3] = {c.diag_tw[0], c.diag_tw[1], c.diag_tw[2]};
+  HIPCHK(launch_spectrum(c, nek, c.diag_Z, tw, c.diag_part, c.diag_out));
+  HIPCHK(hipMemcpyAsync(ek, c.diag_out, sizeof(double) * nek, hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
   return MFC_OK;
 }
 

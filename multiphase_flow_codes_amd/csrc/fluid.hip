@@ -107,6 +107,7 @@ __global__ void __launch_bounds__(256)
     const R c1 = (R)fa.phx[2 * x], s1 = (R)fa.phx[2 * x + 1];
     const R c2 = (R)fa.phx[2 * x2], s2 = (R)fa.phx[2 * x2 + 1];
     R rho, ux, uy, uz;
+    R mx = (R)fa.bfx, my = (R)fa.bfy, mz = (R)fa.bfz;   // force of the macrovar moments
     if (ARR) {
       rho = (R)rho_a[n];
       ux = (R)ux_a[n];
@@ -115,7 +116,6 @@ __global__ void __launch_bounds__(256)
     } else {
       R jx, jy, jz;
       density_momentum<R>(f, rho, jx, jy, jz);
-      R mx = (R)fa.bfx, my = (R)fa.bfy, mz = (R)fa.bfz;
       if (fa.gmac) {
         R gx, gy, gz;
         force_row<R>(fa.gmac + 18ull * row, c1, s1, c2, s2, gx, gy, gz);
@@ -135,7 +135,12 @@ __global__ void __launch_bounds__(256)
       Fy += gy;
       Fz += gz;
     }
-    mrt_collide<R>(f, rho, ux, uy, uz, Fx, Fy, Fz, k);
+    // FP32 arithmetic takes the conserved corrections in closed form; FP64
+    // keeps the reference's own arithmetic
+    if (!ARR && sizeof(R) == 4)
+      mrt_collide<R, true>(f, rho, ux, uy, uz, Fx, Fy, Fz, k, mx, my, mz);
+    else
+      mrt_collide<R, false>(f, rho, ux, uy, uz, Fx, Fy, Fz, k);
   }
 #pragma unroll
   for (int i = 0; i < NPOP; ++i) __builtin_nontemporal_store((T)f[i], &B[(uint64_t)i * g.ncell + n]);
@@ -413,6 +418,8 @@ hipError_t launch_collide(Ctx& c, bool arrays) {
   ForceArgs fa = force_args(c);
   if (c.prec == MFC_PREC_FP64)
     collide_dispatch<double, double>(c, arrays, mask, fa);
+  else if (c.prec == MFC_PREC_FP32_F64MATH)
+    collide_dispatch<float, double>(c, arrays, mask, fa);
   else
     collide_dispatch<float, float>(c, arrays, mask, fa);
   return hipGetLastError();

@@ -99,7 +99,7 @@ struct Ctx {
   mfc_params p{};
   Grid g{};
   CollideConsts cc{};
-  int prec = 0;                 // 0: FP64 populations, 1: FP32
+  int prec = 0;                 // MFC_PREC_*: FP64, FP32, or FP32 storage with FP64 collision
   size_t tsize = 8;
   hipStream_t stream = nullptr;
   bool own_stream = false;
@@ -128,6 +128,12 @@ struct Ctx {
   int timing = 0;
   std::vector<hipEvent_t> tpool;
   int tpool_used = 0;
+  // diagnostics scratch, allocated on first use
+  double* diag_part = nullptr;      // per-block partials
+  double* diag_out = nullptr;       // results
+  int diag_nek = 0;                 // shells the partials were sized for
+  double2* diag_Z = nullptr;        // complex FP64 field
+  double2* diag_tw[3] = {nullptr, nullptr, nullptr};   // exp(-2 pi i j / n) per axis
 };
 
 // ---- launchers (fluid.hip) ----
@@ -157,5 +163,11 @@ hipError_t launch_filling(Ctx& c, bool find = true);
 hipError_t launch_initpartvel(Ctx& c);
 hipError_t launch_maps_to_host_layout(Ctx& c, int which, int32_t* ib, int32_t* is);
 hipError_t launch_links_dense(Ctx& c, int32_t* ilink, int32_t* mlink, double* alink);
+
+// ---- launchers (diag.hip) ----
+int diag_blocks();
+hipError_t launch_field_sums(Ctx& c, double* part, double* out);
+hipError_t launch_spectrum(Ctx& c, int nek, double2* Z, const double2* const tw[3], double* part,
+                           double* acc);
 
 }  // namespace mfc

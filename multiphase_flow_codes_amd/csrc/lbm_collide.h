@@ -79,22 +79,18 @@ MFC_HD void forcing_term(R (&Fb)[NPOP], R ux, R uy, R uz, R Fx, R Fy, R Fz, R w1
   }
 }
 
-// In-place MRT collision of one fluid node.
-//   f        post-streaming populations (deviations), overwritten with the
-//            post-collision populations
+// Corrections corr = M^T D^-1 dm of the collision (see the header comment)
+// from the class sums of g = f + Fbar/2.
 //   rho, u*  the stored density and momentum the reference reads from its
 //            rho/ux/uy/uz arrays
-//   F*       the forcing field at this node for the current step
-template <typename R>
-MFC_HD void mrt_collide(R (&f)[NPOP], R rho, R ux, R uy, R uz, R Fx, R Fy, R Fz,
-                        const CollideConsts& k) {
-  R Fb[NPOP];
-  forcing_term<R>(Fb, ux, uy, uz, Fx, Fy, Fz, (R)k.ww1, (R)k.ww2);
-  R g[NPOP];
-#pragma unroll
-  for (int i = 0; i < NPOP; ++i) g[i] = f[i] + (R)0.5 * Fb[i];
-
-  const ClassSums<R> s = class_sums(g);
+// EXACT_CONS: rho and u are known to be the macrovar moments of f itself,
+// rho = sum f and u = sum c f + Fm/2 (Fm: the force macrovar used).  Then the
+// conserved corrections are exactly dR = 0 and dJ = (F - Fm)/20, since
+// sum Fbar = 0 and sum c Fbar = F; taking them in closed form avoids the
+// cancellation of two rounded 19-term sums.
+template <typename R, bool EXACT_CONS>
+MFC_HD void mrt_corrections(const ClassSums<R>& s, R rho, R ux, R uy, R uz, R Fx, R Fy, R Fz,
+                            const CollideConsts& k, R mFx, R mFy, R mFz, R (&corr)[NPOP]) {
   const R sa = s.sx + s.sy + s.sz;
   const R exy = s.xy_e + s.xy_o, exz = s.xz_e + s.xz_o, eyz = s.yz_e + s.yz_o;
   const R se = exy + exz + eyz;
@@ -134,10 +130,16 @@ MFC_HD void mrt_collide(R (&f)[NPOP], R rho, R ux, R uy, R uz, R Fx, R Fy, R Fz,
   const R eq_pww = uy2 - uz2;
 
   // moment corrections dm, already divided by the row norms
-  const R dR = (m_rho - rho) * (R)(1.0 / 19.0);
+  R dR = (m_rho - rho) * (R)(1.0 / 19.0);
+  R dJx = (m_jx - ux) * (R)0.1, dJy = (m_jy - uy) * (R)0.1, dJz = (m_jz - uz) * (R)0.1;
+  if (EXACT_CONS) {
+    dR = (R)0;
+    dJx = (Fx - mFx) * (R)0.05;
+    dJy = (Fy - mFy) * (R)0.05;
+    dJz = (Fz - mFz) * (R)0.05;
+  }
   const R dE = (R)k.s1 * (m_e - eq_e) * (R)(1.0 / 2394.0);
   const R dP = (R)k.s2 * (m_eps - eq_eps) * (R)(1.0 / 252.0);
-  const R dJx = (m_jx - ux) * (R)0.1, dJy = (m_jy - uy) * (R)0.1, dJz = (m_jz - uz) * (R)0.1;
   const R dQx = (R)k.s4 * (m_qx - eq_q * ux) * (R)(1.0 / 40.0);
   const R dQy = (R)k.s4 * (m_qy - eq_q * uy) * (R)(1.0 / 40.0);
   const R dQz = (R)k.s4 * (m_qz - eq_q * uz) * (R)(1.0 / 40.0);
@@ -164,7 +166,6 @@ MFC_HD void mrt_collide(R (&f)[NPOP], R rho, R ux, R uy, R uz, R Fx, R Fy, R Fz,
   const R exz0 = base_e + dXX + dPX - dWW - dPW;
   const R eyz0 = base_e - (R)2 * dXX - (R)2 * dPX;
 
-  R corr[NPOP];
   corr[0] = dR - (R)30 * dE + (R)12 * dP;
   corr[1] = cxx + ax; corr[2] = cxx - ax;
   corr[3] = cyy + ay; corr[4] = cyy - ay;
@@ -184,7 +185,23 @@ MFC_HD void mrt_collide(R (&f)[NPOP], R rho, R ux, R uy, R uz, R Fx, R Fy, R Fz,
   corr[16] = eyz0 - by + bz - dSyz - dMy - dMz;
   corr[17] = eyz0 + by - bz - dSyz + dMy + dMz;
   corr[18] = eyz0 - by - bz + dSyz - dMy + dMz;
+}
 
+// In-place MRT collision of one fluid node.
+//   f        post-streaming populations (deviations), overwritten with the
+//            post-collision populations
+//   F*       the forcing field at this node for the current step
+// (rho, u*, EXACT_CONS and mF* as for mrt_corrections)
+template <typename R, bool EXACT_CONS = false>
+MFC_HD void mrt_collide(R (&f)[NPOP], R rho, R ux, R uy, R uz, R Fx, R Fy, R Fz,
+                        const CollideConsts& k, R mFx = 0, R mFy = 0, R mFz = 0) {
+  R Fb[NPOP];
+  forcing_term<R>(Fb, ux, uy, uz, Fx, Fy, Fz, (R)k.ww1, (R)k.ww2);
+  R g[NPOP];
+#pragma unroll
+  for (int i = 0; i < NPOP; ++i) g[i] = f[i] + (R)0.5 * Fb[i];
+  R corr[NPOP];
+  mrt_corrections<R, EXACT_CONS>(class_sums(g), rho, ux, uy, uz, Fx, Fy, Fz, k, mFx, mFy, mFz, corr);
 #pragma unroll
   for (int i = 0; i < NPOP; ++i) f[i] = (g[i] - corr[i]) + (R)0.5 * Fb[i];
 }

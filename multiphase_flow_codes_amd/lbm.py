@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(HERE, "libmfc_lbm.so")
 MFC_OK = 0
 ERRORS = {1: "MFC_ERR_ARG", 2: "MFC_ERR_HIP", 3: "MFC_ERR_STATE", 4: "MFC_ERR_OVERLAP",
           5: "MFC_ERR_CAPACITY", 6: "MFC_ERR_NONFINITE", 7: "MFC_ERR_NODEVICE"}
-PREC = {"fp64": 0, "fp32": 1}
+PREC = {"fp64": 0, "fp32": 1, "fp32x": 2}   # fp32x: FP32 storage, FP64 collision
 PFIELDS = {"ypglb": 0, "ypglbp": 1, "wp": 2, "wpp": 3, "omgp": 4, "omgpp": 5, "fHIp": 6,
            "flubp": 7, "torqp": 8, "forcepp": 9, "torqpp": 10, "dwdt": 11, "domgdt": 12,
            "thetap": 13}
@@ -94,6 +94,8 @@ SIGNATURES = {
     "mfc_get_maps": [C.c_void_p, C.c_int32, _I, _I],
     "mfc_get_links": [C.c_void_p, _I, _I, _D],
     "mfc_step": [C.c_void_p, C.c_int32, C.c_int32],
+    "mfc_diag_sums": [C.c_void_p, _D],
+    "mfc_diag_spectrum": [C.c_void_p, C.c_int32, _D],
     "mfc_set_timing": [C.c_void_p, C.c_int32],
     "mfc_get_timing": [C.c_void_p, _D, C.POINTER(C.c_int64)],
 }
@@ -293,6 +295,19 @@ class Lattice:
 
     def step(self, istep: int, nsteps: int = 1):
         self._chk(lib().mfc_step(self.h, int(istep), int(nsteps)))
+
+    def diag_sums(self) -> dict:
+        """FP64 sums over all nodes of the macrovar fields (device reduction)."""
+        out = np.zeros(6)
+        self._chk(lib().mfc_diag_sums(self.h, _dp(out)))
+        return {"rho": out[0], "ux": out[1], "uy": out[2], "uz": out[3], "u2": out[4],
+                "u2max": out[5]}
+
+    def diag_spectrum(self, nek: int) -> np.ndarray:
+        """Kinetic-energy spectrum E(ik), ik = 1..nek, computed on the device."""
+        ek = np.zeros(int(nek))
+        self._chk(lib().mfc_diag_spectrum(self.h, int(nek), _dp(ek)))
+        return ek
 
     def set_timing(self, on: bool):
         self._chk(lib().mfc_set_timing(self.h, int(bool(on))))
