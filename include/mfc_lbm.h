@@ -161,6 +161,33 @@ int mfc_diag_sums(mfc_ctx* ctx, double* out6);
  * field of 16 bytes per node, allocated on first use. */
 int mfc_diag_spectrum(mfc_ctx* ctx, int32_t nek, double* ek);
 
+/* ---- ensemble engine ------------------------------------------------------- */
+/* No counterpart in the reference, which runs one box per job.  For batched
+ * and serving use: many independent boxes, each a context created and
+ * initialised with the calls above (different sizes, spheres, rates and body
+ * forces; one precision and wall setting per ensemble; body-force driving
+ * only, xkf = 0), advanced together with one grid per step stage through
+ * per-box argument and block-offset tables.  A member steps bitwise as it
+ * would alone (mfc_step).  While a box is a member its calls are queued on the
+ * ensemble's stream. */
+typedef struct mfc_ens mfc_ens;
+int mfc_ens_create(mfc_ens** out);
+int mfc_ens_destroy(mfc_ens* e);      /* members are released, not destroyed */
+const char* mfc_ens_last_error(const mfc_ens* e);
+/* Join at the end of a step (after macrovar, and beads_links if it has
+ * spheres); istep = the number of the member's next step. */
+int mfc_ens_add(mfc_ens* e, mfc_ctx* box, int32_t istep);
+int mfc_ens_remove(mfc_ens* e, mfc_ctx* box);
+int mfc_ens_size(const mfc_ens* e);
+int mfc_ens_step(mfc_ens* e, int32_t nsteps);
+/* Waits for queued work; returns the first pending fault of any member. */
+int mfc_ens_sync(mfc_ens* e);
+/* Threads per collide block (64, 128 or 256), or chosen by timing each on the
+ * current members (they advance 3 * steps steps); ms_per_step3 may be null. */
+int mfc_ens_set_block(mfc_ens* e, int32_t bs);
+int mfc_ens_autotune(mfc_ens* e, int32_t steps, int32_t* chosen, double* ms_per_step3);
+void* mfc_ens_get_stream(mfc_ens* e);
+
 int mfc_set_timing(mfc_ctx* ctx, int32_t on);
 int mfc_get_timing(mfc_ctx* ctx, double* total_ms, int64_t* launches);
 

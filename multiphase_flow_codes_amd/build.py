@@ -26,6 +26,7 @@ SOURCES = {
     "fluid.hip": [],
     "particles.hip": ["-ffp-contract=off"],
     "diag.hip": [],
+    "ensemble.hip": [],
 }
 
 

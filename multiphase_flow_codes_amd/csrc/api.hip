@@ -22,9 +22,6 @@
 
 using namespace mfc;
 
-struct mfc_ctx {
-  Ctx c;
-};
 
 namespace {
 
@@ -359,6 +356,7 @@ int mfc_create(const mfc_params* p, mfc_ctx** out) {
 int mfc_destroy(mfc_ctx* ctx) {
   NEEDCTX;
   Ctx& c = ctx->c;
+  if (c.ens) return fail(ctx, MFC_ERR_STATE, "box is an ensemble member: remove it first");
   if (c.stream) hipStreamSynchronize(c.stream);
   auto F = [](void* p) {
     if (p) hipFree(p);

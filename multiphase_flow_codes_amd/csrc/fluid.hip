@@ -13,15 +13,10 @@
 #include "internal.h"
 #include "lbm_access.h"
 #include "lbm_collide.h"
+#include "step_args.h"
 
 namespace mfc {
 
-struct ForceArgs {
-  const double* gcur;   // forcing table of the current step (may be null)
-  const double* gmac;   // forcing table captured by the last macrovar (may be null)
-  const double* phx;    // (cos,sin)(2 pi m / nx)
-  double bfx, bfy, bfz; // uniform body force
-};
 
 // Forcing field at node x of row `row` from a per-row table
 //   F_c = Re G_c0 + 2 Re(e1 G_c1) + 2 Re(e2 G_c2),  e_k = exp(i 2 pi k x / nx),
@@ -60,6 +55,7 @@ __device__ __forceinline__ void force_row(const double* __restrict__ t, R c1, R 
   Fz = v[2];
 }
 
+
 // Fused stream + macro + forcing + MRT collision.
 //
 // Launch: blockIdx.z = z, blockIdx.y = y, blockIdx.x * blockDim.x + lane = x.
@@ -73,14 +69,17 @@ __device__ __forceinline__ void force_row(const double* __restrict__ t, R c1, R 
 // otherwise recomputed from the pulled populations plus half the forcing
 // captured at the last macrovar, which is exactly what those arrays hold.
 template <typename T, typename R, bool ARR, bool WALLS>
-__global__ void __launch_bounds__(256)
-    k_collide(const T* __restrict__ A, T* __restrict__ B, const uint32_t* __restrict__ mask, Grid g,
-              CollideConsts k, ForceArgs fa, const double* __restrict__ rho_a,
-              const double* __restrict__ ux_a, const double* __restrict__ uy_a,
-              const double* __restrict__ uz_a) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y, z = blockIdx.z;
-  if (x >= g.nx) return;
+__device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y, int z) {
+  const T* __restrict__ A = (const T*)a.A;
+  T* __restrict__ B = (T*)a.B;
+  const uint32_t* __restrict__ mask = a.mask;
+  const Grid& g = a.g;
+  const CollideConsts& k = a.k;
+  const ForceArgs& fa = a.fa;
+  const double* __restrict__ rho_a = a.rho;
+  const double* __restrict__ ux_a = a.ux;
+  const double* __restrict__ uy_a = a.uy;
+  const double* __restrict__ uz_a = a.uz;
   const uint32_t nx = (uint32_t)g.nx, ny = (uint32_t)g.ny;
   const uint32_t ym = y == 0 ? ny - 1 : y - 1, yp = y == g.ny - 1 ? 0 : y + 1;
   const uint32_t zm = z == 0 ? g.nz - 1 : z - 1, zp = z == g.nz - 1 ? 0 : z + 1;
@@ -144,6 +143,42 @@ __global__ void __launch_bounds__(256)
   }
 #pragma unroll
   for (int i = 0; i < NPOP; ++i) __builtin_nontemporal_store((T)f[i], &B[(uint64_t)i * g.ncell + n]);
+}
+
+template <typename T, typename R, bool ARR, bool WALLS>
+__global__ void __launch_bounds__(256) k_collide(CollideArgs a) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= a.g.nx) return;
+  collide_node<T, R, ARR, WALLS>(a, x, blockIdx.y, blockIdx.z);
+}
+
+// Several boxes in one grid (ensemble engine): block b belongs to box c with
+// off[c] <= b < off[c+1]; the box's nodes are numbered x fastest and taken
+// BS per block, so consecutive lanes still read consecutive x (and rows
+// continue into the next row without a gap).
+__device__ __forceinline__ int case_of_block(const int* __restrict__ off, int nc, int b) {
+  int lo = 0, hi = nc;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= b) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <typename T, typename R, bool WALLS, int BS>
+__global__ void __launch_bounds__(BS)
+    kb_collide(const CollideArgs* __restrict__ t, const int* __restrict__ off, int nc) {
+  const int c = case_of_block(off, nc, (int)blockIdx.x);
+  const CollideArgs a = t[c];
+  const uint64_t n = (uint64_t)((int)blockIdx.x - off[c]) * BS + threadIdx.x;
+  if (n >= a.g.ncell) return;
+  const uint32_t nx = (uint32_t)a.g.nx, ny = (uint32_t)a.g.ny;
+  const uint32_t x = (uint32_t)(n % nx);
+  const uint32_t r = (uint32_t)(n / nx);
+  if (a.arr)
+    collide_node<T, R, true, WALLS>(a, (int)x, (int)(r % ny), (int)(r / ny));
+  else
+    collide_node<T, R, false, WALLS>(a, (int)x, (int)(r % ny), (int)(r / ny));
 }
 
 // Forcing table of one step: per (y,z) row the nine partial sums
@@ -390,38 +425,45 @@ static void row_launch(const Grid& g, dim3& grid, dim3& blk) {
   grid = dim3((g.nx + bx - 1) / bx, g.ny, g.nz);
 }
 
+// Arguments of one box's collide launch (also used by the ensemble engine).
+CollideArgs collide_args(Ctx& c) {
+  CollideArgs a;
+  a.A = c.pop[c.cur];
+  a.B = c.pop[c.cur ^ 1];
+  a.mask = (c.pt.np > 0 && c.pt.map_np[c.pt.map_cur] > 0) ? c.pt.mask[c.pt.map_cur] : nullptr;
+  a.g = c.g;
+  a.k = c.cc;
+  a.fa = force_args(c);
+  a.rho = c.rho; a.ux = c.ux; a.uy = c.uy; a.uz = c.uz;
+  a.arr = 0;
+  return a;
+}
+
 template <typename T, typename R>
-static void collide_dispatch(Ctx& c, bool arrays, const uint32_t* mask, ForceArgs fa) {
-  const T* A = (const T*)c.pop[c.cur];
-  T* B = (T*)c.pop[c.cur ^ 1];
+static void collide_dispatch(Ctx& c, bool arrays, const CollideArgs& a) {
   dim3 grid, blk;
   row_launch(c.g, grid, blk);
   if (arrays) {
     if (c.g.walls_y)
-      hipLaunchKernelGGL((k_collide<T, R, true, true>), grid, blk, 0, c.stream, A, B, mask, c.g, c.cc,
-                         fa, c.rho, c.ux, c.uy, c.uz);
+      hipLaunchKernelGGL((k_collide<T, R, true, true>), grid, blk, 0, c.stream, a);
     else
-      hipLaunchKernelGGL((k_collide<T, R, true, false>), grid, blk, 0, c.stream, A, B, mask, c.g, c.cc,
-                         fa, c.rho, c.ux, c.uy, c.uz);
+      hipLaunchKernelGGL((k_collide<T, R, true, false>), grid, blk, 0, c.stream, a);
   } else {
     if (c.g.walls_y)
-      hipLaunchKernelGGL((k_collide<T, R, false, true>), grid, blk, 0, c.stream, A, B, mask, c.g,
-                         c.cc, fa, nullptr, nullptr, nullptr, nullptr);
+      hipLaunchKernelGGL((k_collide<T, R, false, true>), grid, blk, 0, c.stream, a);
     else
-      hipLaunchKernelGGL((k_collide<T, R, false, false>), grid, blk, 0, c.stream, A, B, mask, c.g,
-                         c.cc, fa, nullptr, nullptr, nullptr, nullptr);
+      hipLaunchKernelGGL((k_collide<T, R, false, false>), grid, blk, 0, c.stream, a);
   }
 }
 
 hipError_t launch_collide(Ctx& c, bool arrays) {
-  const uint32_t* mask = (c.pt.np > 0 && c.pt.map_np[c.pt.map_cur] > 0) ? c.pt.mask[c.pt.map_cur] : nullptr;
-  ForceArgs fa = force_args(c);
+  const CollideArgs a = collide_args(c);
   if (c.prec == MFC_PREC_FP64)
-    collide_dispatch<double, double>(c, arrays, mask, fa);
+    collide_dispatch<double, double>(c, arrays, a);
   else if (c.prec == MFC_PREC_FP32_F64MATH)
-    collide_dispatch<float, double>(c, arrays, mask, fa);
+    collide_dispatch<float, double>(c, arrays, a);
   else
-    collide_dispatch<float, float>(c, arrays, mask, fa);
+    collide_dispatch<float, float>(c, arrays, a);
   return hipGetLastError();
 }
 
@@ -536,6 +578,44 @@ hipError_t launch_force_field(Ctx& c, int slot, double* fx, double* fy, double* 
   ForceArgs fa = force_args(c);
   fa.gcur = c.ft.live[slot] ? c.ft.g[slot] : nullptr;
   hipLaunchKernelGGL(k_force_field, dim3(nblocks(c.g.ncell)), dim3(256), 0, c.stream, c.g, fa, fx, fy, fz);
+  return hipGetLastError();
+}
+
+}  // namespace mfc
+
+namespace mfc {
+
+template <int BS>
+static void batched_collide_bs(int prec, int walls, const void* t, const int* off, int nc, int blocks,
+                               hipStream_t s) {
+  const CollideArgs* tab = (const CollideArgs*)t;
+  if (prec == MFC_PREC_FP64) {
+    if (walls)
+      hipLaunchKernelGGL((kb_collide<double, double, true, BS>), dim3(blocks), dim3(BS), 0, s, tab, off, nc);
+    else
+      hipLaunchKernelGGL((kb_collide<double, double, false, BS>), dim3(blocks), dim3(BS), 0, s, tab, off, nc);
+  } else if (prec == MFC_PREC_FP32_F64MATH) {
+    if (walls)
+      hipLaunchKernelGGL((kb_collide<float, double, true, BS>), dim3(blocks), dim3(BS), 0, s, tab, off, nc);
+    else
+      hipLaunchKernelGGL((kb_collide<float, double, false, BS>), dim3(blocks), dim3(BS), 0, s, tab, off, nc);
+  } else {
+    if (walls)
+      hipLaunchKernelGGL((kb_collide<float, float, true, BS>), dim3(blocks), dim3(BS), 0, s, tab, off, nc);
+    else
+      hipLaunchKernelGGL((kb_collide<float, float, false, BS>), dim3(blocks), dim3(BS), 0, s, tab, off, nc);
+  }
+}
+
+hipError_t launch_batched_collide(int prec, int walls, int bs, const void* t, const int* off, int nc,
+                                  int blocks, hipStream_t s) {
+  if (nc <= 0 || blocks <= 0) return hipSuccess;
+  switch (bs) {
+    case 64: batched_collide_bs<64>(prec, walls, t, off, nc, blocks, s); break;
+    case 128: batched_collide_bs<128>(prec, walls, t, off, nc, blocks, s); break;
+    case 256: batched_collide_bs<256>(prec, walls, t, off, nc, blocks, s); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -134,6 +134,7 @@ struct Ctx {
   int diag_nek = 0;                 // shells the partials were sized for
   double2* diag_Z = nullptr;        // complex FP64 field
   double2* diag_tw[3] = {nullptr, nullptr, nullptr};   // exp(-2 pi i j / n) per axis
+  void* ens = nullptr;              // the ensemble this box is a member of (its stream is on loan)
 };
 
 // ---- launchers (fluid.hip) ----
@@ -171,3 +172,9 @@ hipError_t launch_spectrum(Ctx& c, int nek, double2* Z, const double2* const tw[
                            double* acc);
 
 }  // namespace mfc
+
+// the opaque handle of the C ABI
+struct mfc_ctx {
+  mfc::Ctx c;
+};
+

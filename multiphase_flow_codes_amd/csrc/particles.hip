@@ -17,6 +17,7 @@
 #include "internal.h"
 #include "lbm_access.h"
 #include "lbm_particle_math.h"
+#include "step_args.h"
 
 namespace mfc {
 
@@ -26,16 +27,6 @@ constexpr int PB = 256;  // threads per particle workgroup
 // workgroups.  Segment s of particle p is block p * nseg + s.
 constexpr int SLAB = 4;
 
-struct PGeom {
-  int nx, ny, nz, walls_y;
-  double rad, radp2, r2;
-  // (r + 1.5)^2: a node farther than r + 1.5 from the centre is neither
-  // solid nor has a link, since every lattice vector is shorter than 1.5
-  // (the margin over sqrt(2) dwarfs any rounding of the squared distances)
-  double rlink2;
-  // (r + 0.5)^2: no solid node lies farther out than this
-  double rsolid2;
-};
 
 struct AxisRange {
   int lo, cnt;
@@ -284,8 +275,7 @@ __global__ void __launch_bounds__(PB)
 // With `ovl` set the first segment of each particle also performs the overlap
 // check of beads_redistribute (partlib.f90:2631-2646) on the freshly wrapped
 // positions.  Dynamic LDS: seg_cap node indices.
-__global__ void __launch_bounds__(PB)
-    k_links_scan(const double* __restrict__ ypglb, int np, int nseg, int seg_cap, PGeom G,
+__device__ __forceinline__ void links_scan_body(int b_, const double* __restrict__ ypglb, int np, int nseg, int seg_cap, PGeom G,
                  int32_t* __restrict__ isn, uint32_t* __restrict__ mask,
                  uint32_t* __restrict__ sol_list, int32_t* __restrict__ sol_cnt,
                  uint32_t* __restrict__ sh_node, uint32_t* __restrict__ sh_hits,
@@ -296,7 +286,7 @@ __global__ void __launch_bounds__(PB)
   __shared__ int wt[2][PB / 64][2];
   __shared__ int sh[PB / 64];
   __shared__ int hit;
-  const int b = blockIdx.x;
+  const int b = b_;
   const int p = b / nseg, s = b - p * nseg;
   if (p >= np) return;
   if (ovl && s == 0) {
@@ -377,8 +367,7 @@ __global__ void __launch_bounds__(PB)
 
 // Exclusive scan of the per-segment link counts (one workgroup of 1024):
 // each thread sums a contiguous run, the run totals are scanned in LDS.
-__global__ void __launch_bounds__(1024)
-    k_scan_counts(const int32_t* __restrict__ cnt, const int32_t* __restrict__ scnt, int nsegs,
+__device__ __forceinline__ void scan_counts_body(int b_, const int32_t* __restrict__ cnt, const int32_t* __restrict__ scnt, int nsegs,
                   int64_t* __restrict__ off, int64_t cap, int32_t* __restrict__ status,
                   int32_t* __restrict__ fill_count) {
   if (threadIdx.x == 0) *fill_count = 0;   // the next beads_filling starts an empty list
@@ -416,18 +405,6 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
-struct FillArgs {
-  const double *ypglbp, *wp, *wpp, *omgp, *omgpp;
-  const int32_t* isn_old;
-  const uint32_t *mask_new, *mask_old;
-  const uint32_t* sol_old;    // solid lists of the previous map build
-  const int32_t* sol_cnt_old;
-  int seg_cap;
-  double rad, rho0, ww0, ww1, ww2;
-  uint64_t* list;       // (particle << 32) | node of every node to refill
-  int32_t* count;       // list length (reset by k_scan_counts of the links pass)
-  int64_t cap;
-};
 __device__ void fill_find_segment(int b, const FillArgs& fa, int nseg, int32_t* __restrict__ status);
 
 // Second half of beads_links: expand each segment's (node, direction bits)
@@ -436,8 +413,7 @@ __device__ void fill_find_segment(int b, const FillArgs& fa, int nseg, int32_t* 
 // segment (partlib.f90:1013-1024), one link per lane, with the node geometry
 // rebuilt exactly as the scan saw it.  Blocks past the link segments run the
 // first pass of beads_filling (the new map is complete at this point).
-__global__ void __launch_bounds__(PB)
-    k_links_expand(const double* __restrict__ ypglb, int np, int nseg, int seg_cap, PGeom G,
+__device__ __forceinline__ void links_expand_body(int b_, const double* __restrict__ ypglb, int np, int nseg, int seg_cap, PGeom G,
                    const uint32_t* __restrict__ sh_node, const uint32_t* __restrict__ sh_hits,
                    const int32_t* __restrict__ sh_cnt, const int64_t* __restrict__ off, int64_t cap,
                    uint32_t* __restrict__ lk_node, uint8_t* __restrict__ lk_dir,
@@ -445,7 +421,7 @@ __global__ void __launch_bounds__(PB)
                    FillArgs fa) {
   __shared__ BoxInfo B;
   __shared__ int sh[PB / 64];
-  const int b = blockIdx.x;
+  const int b = b_;
   if (b >= nsegs) {
     fill_find_segment(b - nsegs, fa, nseg, status);
     return;
@@ -497,10 +473,6 @@ __global__ void __launch_bounds__(PB)
   if (alpha_bad) atomicAdd(&status[ST_ALPHA_FAULT], alpha_bad);
 }
 
-struct LubArgs {
-  double mingap, fpc, stf0, stf1;
-  double* flubp;
-};
 
 // beads_lubforce (partlib.f90:2211-2292): one workgroup sums the pair forces
 // of particle a; each pair is evaluated as the reference does, from the lower
@@ -560,8 +532,7 @@ __device__ __forceinline__ bool bit_at(const uint32_t* __restrict__ mask, uint32
 // into seg_ft.  All links read one snapshot of the post-collision field; the
 // values are scattered into the solid nodes by k_bb_scatter afterwards.
 template <typename T>
-__global__ void __launch_bounds__(PB)
-    k_bounce_back(const T* __restrict__ A, Grid g, PGeom G, const uint32_t* __restrict__ mask,
+__device__ __forceinline__ void bounce_back_body(int b_, const T* __restrict__ A, Grid g, PGeom G, const uint32_t* __restrict__ mask,
                   int nseg, const int64_t* __restrict__ off, int64_t cap,
                   const uint32_t* __restrict__ lk_node, const uint8_t* __restrict__ lk_dir,
                   const double* __restrict__ lk_alpha, double* __restrict__ lk_out,
@@ -570,7 +541,7 @@ __global__ void __launch_bounds__(PB)
                   double* __restrict__ seg_ft, int32_t* __restrict__ status, int nsegs, int np,
                   LubArgs lub) {
   __shared__ double red[6][PB / 64];
-  const int b = blockIdx.x;
+  const int b = b_;
   if (b >= nsegs) {
     // blocks past the link segments: beads_lubforce of one particle
     lub_block(b - nsegs, ypglb, np, G, lub);
@@ -666,14 +637,13 @@ __global__ void __launch_bounds__(PB)
 // solid neighbour x + c (partlib.f90:1605-1621); the first segment of each
 // particle also sums its segments' force and torque into fHIp / torqp.
 template <typename T>
-__global__ void __launch_bounds__(PB)
-    k_bb_scatter(T* __restrict__ A, Grid g, int nseg, const int64_t* __restrict__ off, int64_t cap,
+__device__ __forceinline__ void bb_scatter_body(int b_, T* __restrict__ A, Grid g, int nseg, const int64_t* __restrict__ off, int64_t cap,
                  const uint32_t* __restrict__ lk_node, const uint8_t* __restrict__ lk_dir,
                  const double* __restrict__ lk_out, const double* __restrict__ seg_ft,
                  double* __restrict__ fHIp, double* __restrict__ torqp, int nsegs, int seg_cap,
                  const uint32_t* __restrict__ clr_list, const int32_t* __restrict__ clr_cnt,
                  int32_t* __restrict__ clr_isn, uint32_t* __restrict__ clr_mask) {
-  const int b = blockIdx.x;
+  const int b = b_;
   if (b >= nsegs) {
     // blocks past the link segments: reset one segment of the map buffer the
     // coming beads_links rebuilds (it was built two steps ago)
@@ -705,12 +675,6 @@ __global__ void __launch_bounds__(PB)
   }
 }
 
-struct MoveArgs {
-  double *yp, *ypglb, *ypglbp, *wp, *wpp, *omgp, *omgpp, *fHIp, *flubp, *torqp, *forcep, *forcepp,
-      *torqpp, *dwdt, *domgdt, *thetap;
-  double volp, rhog, amp, aip;
-  double bfx, bfy, bfz;
-};
 
 __device__ __forceinline__ bool wrap_particle(double* yp, double* ypglb, double* ypglbp, int p,
                                               const PGeom& G) {
@@ -734,9 +698,8 @@ __device__ __forceinline__ bool wrap_particle(double* yp, double* ypglb, double*
 // `wrap` set the periodic wrap of beads_redistribute (partlib.f90:2600-2626)
 // follows in the same thread.  Every input is loaded before the first store
 // (the argument arrays are distinct, but the compiler cannot know it).
-__global__ void __launch_bounds__(256)
-    k_move(MoveArgs m, int np, int first, int wrap, PGeom G, int32_t* __restrict__ status) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void move_body(int b_, MoveArgs m, int np, int first, int wrap, PGeom G, int32_t* __restrict__ status) {
+  const int p = b_ * blockDim.x + threadIdx.x;
   if (p >= np) return;
   const double dt = 1.0;
   const double bf[3] = {m.bfx, m.bfy, m.bfz};
@@ -902,15 +865,14 @@ __global__ void __launch_bounds__(PB)
 // (new w, omega; old centre) and the neighbours' mean density, plus the
 // non-equilibrium part.  Written into the pull sources of the node.
 template <typename T, bool WALLS>
-__global__ void __launch_bounds__(256)
-    k_fill_apply(T* __restrict__ A, Grid g, FillArgs fa) {
+__device__ __forceinline__ void fill_apply_body(int b_, T* __restrict__ A, Grid g, FillArgs fa, int nblk) {
   __shared__ double fneq[4][NPOP];
   __shared__ double srho[4][NPOP];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int64_t nwaves = (int64_t)gridDim.x * 4;
+  const int64_t nwaves = (int64_t)nblk * 4;
   const int64_t total = min((int64_t)*fa.count, fa.cap);
   // the trip count is the same for every wavefront of the block (barriers inside)
-  for (int64_t base = (int64_t)blockIdx.x * 4; base < total; base += nwaves) {
+  for (int64_t base = (int64_t)b_ * 4; base < total; base += nwaves) {
     const int64_t e = base + w;
     const bool active = e < total;
     const uint64_t ent = active ? fa.list[e] : 0ull;
@@ -990,6 +952,96 @@ __global__ void __launch_bounds__(256)
     }
     __syncthreads();
   }
+}
+
+// ---------------------------------------------------------------------------
+// Launch forms of the step kernels.  Each kernel body takes its block index;
+// k_* launches one box (arguments by value), kb_* one grid over several boxes
+// at once: block b belongs to box c with off[c] <= b < off[c+1] and reads that
+// box's arguments from the table t (the ensemble engine's per-case tables).
+__device__ __forceinline__ int case_of(const int* __restrict__ off, int nc, int b) {
+  int lo = 0, hi = nc;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= b) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(PB) k_links_scan(ScanArgs a) {
+  links_scan_body((int)blockIdx.x, a.ypglb, a.np, a.nseg, a.seg_cap, a.G, a.isn, a.mask, a.sol_list, a.sol_cnt, a.sh_node, a.sh_hits, a.sh_cnt, a.lk_cnt, a.ovl, a.istep, a.status);
+}
+__global__ void __launch_bounds__(PB)
+    kb_links_scan(const ScanArgs* __restrict__ t, const int* __restrict__ off, int nc) {
+  const int c = case_of(off, nc, (int)blockIdx.x);
+  const ScanArgs a = t[c];
+  links_scan_body((int)blockIdx.x - off[c], a.ypglb, a.np, a.nseg, a.seg_cap, a.G, a.isn, a.mask, a.sol_list, a.sol_cnt, a.sh_node, a.sh_hits, a.sh_cnt, a.lk_cnt, a.ovl, a.istep, a.status);
+}
+
+__global__ void __launch_bounds__(1024) k_scan_counts(ScanCountArgs a) {
+  scan_counts_body((int)blockIdx.x, a.cnt, a.scnt, a.nsegs, a.off, a.cap, a.status, a.fill_count);
+}
+__global__ void __launch_bounds__(1024)
+    kb_scan_counts(const ScanCountArgs* __restrict__ t, const int* __restrict__ off, int nc) {
+  const int c = case_of(off, nc, (int)blockIdx.x);
+  const ScanCountArgs a = t[c];
+  scan_counts_body((int)blockIdx.x - off[c], a.cnt, a.scnt, a.nsegs, a.off, a.cap, a.status, a.fill_count);
+}
+
+__global__ void __launch_bounds__(PB) k_links_expand(ExpandArgs a) {
+  links_expand_body((int)blockIdx.x, a.ypglb, a.np, a.nseg, a.seg_cap, a.G, a.sh_node, a.sh_hits, a.sh_cnt, a.off, a.cap, a.lk_node, a.lk_dir, a.lk_alpha, a.status, a.nsegs, a.fa);
+}
+__global__ void __launch_bounds__(PB)
+    kb_links_expand(const ExpandArgs* __restrict__ t, const int* __restrict__ off, int nc) {
+  const int c = case_of(off, nc, (int)blockIdx.x);
+  const ExpandArgs a = t[c];
+  links_expand_body((int)blockIdx.x - off[c], a.ypglb, a.np, a.nseg, a.seg_cap, a.G, a.sh_node, a.sh_hits, a.sh_cnt, a.off, a.cap, a.lk_node, a.lk_dir, a.lk_alpha, a.status, a.nsegs, a.fa);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(PB) k_bounce_back(BounceArgs a) {
+  bounce_back_body<T>((int)blockIdx.x, (const T*)a.A, a.g, a.G, a.mask, a.nseg, a.off, a.cap, a.lk_node, a.lk_dir, a.lk_alpha, a.lk_out, a.ypglb, a.wp, a.omgp, a.ww1, a.ww2, a.seg_ft, a.status, a.nsegs, a.np, a.lub);
+}
+template <typename T>
+__global__ void __launch_bounds__(PB)
+    kb_bounce_back(const BounceArgs* __restrict__ t, const int* __restrict__ off, int nc) {
+  const int c = case_of(off, nc, (int)blockIdx.x);
+  const BounceArgs a = t[c];
+  bounce_back_body<T>((int)blockIdx.x - off[c], (const T*)a.A, a.g, a.G, a.mask, a.nseg, a.off, a.cap, a.lk_node, a.lk_dir, a.lk_alpha, a.lk_out, a.ypglb, a.wp, a.omgp, a.ww1, a.ww2, a.seg_ft, a.status, a.nsegs, a.np, a.lub);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(PB) k_bb_scatter(ScatterArgs a) {
+  bb_scatter_body<T>((int)blockIdx.x, (T*)a.A, a.g, a.nseg, a.off, a.cap, a.lk_node, a.lk_dir, a.lk_out, a.seg_ft, a.fHIp, a.torqp, a.nsegs, a.seg_cap, a.clr_list, a.clr_cnt, a.clr_isn, a.clr_mask);
+}
+template <typename T>
+__global__ void __launch_bounds__(PB)
+    kb_bb_scatter(const ScatterArgs* __restrict__ t, const int* __restrict__ off, int nc) {
+  const int c = case_of(off, nc, (int)blockIdx.x);
+  const ScatterArgs a = t[c];
+  bb_scatter_body<T>((int)blockIdx.x - off[c], (T*)a.A, a.g, a.nseg, a.off, a.cap, a.lk_node, a.lk_dir, a.lk_out, a.seg_ft, a.fHIp, a.torqp, a.nsegs, a.seg_cap, a.clr_list, a.clr_cnt, a.clr_isn, a.clr_mask);
+}
+
+__global__ void __launch_bounds__(256) k_move(MoveKArgs a) {
+  move_body((int)blockIdx.x, a.m, a.np, a.first, a.wrap, a.G, a.status);
+}
+__global__ void __launch_bounds__(256)
+    kb_move(const MoveKArgs* __restrict__ t, const int* __restrict__ off, int nc) {
+  const int c = case_of(off, nc, (int)blockIdx.x);
+  const MoveKArgs a = t[c];
+  move_body((int)blockIdx.x - off[c], a.m, a.np, a.first, a.wrap, a.G, a.status);
+}
+
+template <typename T, bool WALLS>
+__global__ void __launch_bounds__(256) k_fill_apply(FillApplyArgs a) {
+  fill_apply_body<T, WALLS>((int)blockIdx.x, (T*)a.A, a.g, a.fa, a.nblk);
+}
+template <typename T, bool WALLS>
+__global__ void __launch_bounds__(256)
+    kb_fill_apply(const FillApplyArgs* __restrict__ t, const int* __restrict__ off, int nc) {
+  const int c = case_of(off, nc, (int)blockIdx.x);
+  const FillApplyArgs a = t[c];
+  fill_apply_body<T, WALLS>((int)blockIdx.x - off[c], (T*)a.A, a.g, a.fa, a.nblk);
 }
 
 // trilinear velocity at the particle centres (initpartvel,
@@ -1102,28 +1154,6 @@ static bool make_fill_args(Ctx& c, int old, int neu, FillArgs& fa) {
   return true;
 }
 
-hipError_t launch_links(Ctx& c, int target, int overlap_check, int istep, bool with_find) {
-  Particles& P = c.pt;
-  const PGeom G = pgeom(c);
-  const int nsegs = P.np * P.nseg;
-  hipError_t e = launch_map_clear(c, target);
-  if (e != hipSuccess) return e;
-  if (P.np <= 0) return hipGetLastError();
-  FillArgs fa{};
-  const int nfind = (with_find && make_fill_args(c, P.map_old, target, fa)) ? P.map_np[P.map_old] * P.nseg : 0;
-  hipLaunchKernelGGL(k_links_scan, dim3(nsegs), dim3(PB), sizeof(uint32_t) * (size_t)P.seg_cap,
-                     c.stream, P.ypglb, P.np, P.nseg, P.seg_cap, G, P.isn[target], P.mask[target],
-                     P.sol_list[target], P.sol_cnt[target], P.sh_node, P.sh_hits, P.sh_cnt, P.lk_cnt,
-                     overlap_check, istep, c.d_status);
-  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, c.stream, P.lk_cnt, P.sol_cnt[target],
-                     nsegs, P.lk_off, P.lk_cap, c.d_status, P.fill_count);
-  hipLaunchKernelGGL(k_links_expand, dim3(nsegs + nfind), dim3(PB), 0, c.stream, P.ypglb, P.np,
-                     P.nseg, P.seg_cap, G, P.sh_node, P.sh_hits, P.sh_cnt, P.lk_off, P.lk_cap,
-                     P.lk_node, P.lk_dir, P.lk_alpha, c.d_status, nsegs, fa);
-  P.map_np[target] = P.np;
-  return hipGetLastError();
-}
-
 static LubArgs lub_args(Ctx& c) {
   LubArgs L;
   L.mingap = c.p.mingap;
@@ -1134,30 +1164,123 @@ static LubArgs lub_args(Ctx& c) {
   return L;
 }
 
-template <typename T>
-static void bb_dispatch(Ctx& c, bool fused) {
+// ---- argument builders: one box's arguments of each step kernel ------------
+// (used by the single-box launchers below and by the ensemble engine)
+
+ScanArgs scan_args(Ctx& c, int target, int ovl, int istep) {
   Particles& P = c.pt;
-  T* A = (T*)c.pop[c.cur];
-  const int nsegs = P.np * P.nseg;
-  const int nlub = fused ? P.np : 0;
-  hipLaunchKernelGGL((k_bounce_back<T>), dim3(nsegs + nlub), dim3(PB), 0, c.stream, A, c.g, pgeom(c),
-                     P.mask[P.map_cur], P.nseg, P.lk_off, P.lk_cap, P.lk_node, P.lk_dir, P.lk_alpha,
-                     P.lk_out, P.ypglb, P.wp, P.omgp, c.p.ww1, c.p.ww2, P.seg_ft, c.d_status, nsegs,
-                     P.np, lub_args(c));
-  // the map buffer the next beads_links rebuilds (the previous map stays)
+  ScanArgs a;
+  a.ypglb = P.ypglb; a.np = P.np; a.nseg = P.nseg; a.seg_cap = P.seg_cap; a.G = pgeom(c);
+  a.isn = P.isn[target]; a.mask = P.mask[target]; a.sol_list = P.sol_list[target];
+  a.sol_cnt = P.sol_cnt[target]; a.sh_node = P.sh_node; a.sh_hits = P.sh_hits; a.sh_cnt = P.sh_cnt;
+  a.lk_cnt = P.lk_cnt; a.ovl = ovl; a.istep = istep; a.status = c.d_status;
+  return a;
+}
+
+ScanCountArgs count_args(Ctx& c, int target) {
+  Particles& P = c.pt;
+  ScanCountArgs a;
+  a.cnt = P.lk_cnt; a.scnt = P.sol_cnt[target]; a.nsegs = P.np * P.nseg; a.off = P.lk_off;
+  a.cap = P.lk_cap; a.status = c.d_status; a.fill_count = P.fill_count;
+  return a;
+}
+
+// with_find: the refill search (previous map P.map_old vs `target`) rides in
+// the same grid; *blocks gets the grid size
+ExpandArgs expand_args(Ctx& c, int target, bool with_find, int* blocks) {
+  Particles& P = c.pt;
+  ExpandArgs a;
+  a.ypglb = P.ypglb; a.np = P.np; a.nseg = P.nseg; a.seg_cap = P.seg_cap; a.G = pgeom(c);
+  a.sh_node = P.sh_node; a.sh_hits = P.sh_hits; a.sh_cnt = P.sh_cnt; a.off = P.lk_off; a.cap = P.lk_cap;
+  a.lk_node = P.lk_node; a.lk_dir = P.lk_dir; a.lk_alpha = P.lk_alpha; a.status = c.d_status;
+  a.nsegs = P.np * P.nseg;
+  a.fa = FillArgs{};
+  const int nfind = (with_find && make_fill_args(c, P.map_old, target, a.fa)) ? P.map_np[P.map_old] * P.nseg : 0;
+  *blocks = a.nsegs + nfind;
+  return a;
+}
+
+// fused: the lubrication forces ride in the same grid
+BounceArgs bounce_args(Ctx& c, bool fused, int* blocks) {
+  Particles& P = c.pt;
+  BounceArgs a;
+  a.A = c.pop[c.cur]; a.g = c.g; a.G = pgeom(c); a.mask = P.mask[P.map_cur]; a.nseg = P.nseg;
+  a.off = P.lk_off; a.cap = P.lk_cap; a.lk_node = P.lk_node; a.lk_dir = P.lk_dir;
+  a.lk_alpha = P.lk_alpha; a.lk_out = P.lk_out; a.ypglb = P.ypglb; a.wp = P.wp; a.omgp = P.omgp;
+  a.ww1 = c.p.ww1; a.ww2 = c.p.ww2; a.seg_ft = P.seg_ft; a.status = c.d_status;
+  a.nsegs = P.np * P.nseg; a.np = P.np; a.lub = lub_args(c);
+  *blocks = a.nsegs + (fused ? P.np : 0);
+  return a;
+}
+
+// fused: the map buffer the coming links rebuild (map_cur ^ 1, built two
+// steps ago) is cleared in the same grid, and marked empty
+ScatterArgs scatter_args(Ctx& c, bool fused, int* blocks) {
+  Particles& P = c.pt;
   const int t = P.map_cur ^ 1;
   const int nclr = (fused && P.map_np[t] > 0) ? P.map_np[t] * P.nseg : 0;
-  hipLaunchKernelGGL((k_bb_scatter<T>), dim3(nsegs + nclr), dim3(PB), 0, c.stream, A, c.g, P.nseg,
-                     P.lk_off, P.lk_cap, P.lk_node, P.lk_dir, P.lk_out, P.seg_ft, P.fHIp, P.torqp,
-                     nsegs, P.seg_cap, P.sol_list[t], P.sol_cnt[t], P.isn[t], P.mask[t]);
+  ScatterArgs a;
+  a.A = c.pop[c.cur]; a.g = c.g; a.nseg = P.nseg; a.off = P.lk_off; a.cap = P.lk_cap;
+  a.lk_node = P.lk_node; a.lk_dir = P.lk_dir; a.lk_out = P.lk_out; a.seg_ft = P.seg_ft;
+  a.fHIp = P.fHIp; a.torqp = P.torqp; a.nsegs = P.np * P.nseg; a.seg_cap = P.seg_cap;
+  a.clr_list = P.sol_list[t]; a.clr_cnt = P.sol_cnt[t]; a.clr_isn = P.isn[t]; a.clr_mask = P.mask[t];
   if (nclr) P.map_np[t] = 0;
+  *blocks = a.nsegs + nclr;
+  return a;
+}
+
+MoveKArgs move_args(Ctx& c, int first, int wrap, int* blocks) {
+  Particles& P = c.pt;
+  MoveKArgs a;
+  MoveArgs& m = a.m;
+  m.yp = P.yp; m.ypglb = P.ypglb; m.ypglbp = P.ypglbp;
+  m.wp = P.wp; m.wpp = P.wpp; m.omgp = P.omgp; m.omgpp = P.omgpp;
+  m.fHIp = P.fHIp; m.flubp = P.flubp; m.torqp = P.torqp; m.forcep = P.forcep;
+  m.forcepp = P.forcepp; m.torqpp = P.torqpp; m.dwdt = P.dwdt; m.domgdt = P.domgdt;
+  m.thetap = P.thetap;
+  m.volp = c.p.volp; m.rhog = c.p.rhog; m.amp = c.p.amp; m.aip = c.p.aip;
+  m.bfx = c.p.body_force[0]; m.bfy = c.p.body_force[1]; m.bfz = c.p.body_force[2];
+  a.np = P.np; a.first = first; a.wrap = wrap; a.G = pgeom(c); a.status = c.d_status;
+  *blocks = (P.np + 255) / 256;
+  return a;
+}
+
+// false if there is nothing to refill
+bool fill_apply_args(Ctx& c, int nblk, FillApplyArgs& a) {
+  Particles& P = c.pt;
+  if (!make_fill_args(c, P.map_old, P.map_cur, a.fa)) return false;
+  a.A = c.pop[c.cur]; a.g = c.g; a.nblk = nblk;
+  return true;
+}
+
+// ---- single-box launchers -------------------------------------------------------
+hipError_t launch_links(Ctx& c, int target, int overlap_check, int istep, bool with_find) {
+  Particles& P = c.pt;
+  const int nsegs = P.np * P.nseg;
+  hipError_t e = launch_map_clear(c, target);
+  if (e != hipSuccess) return e;
+  if (P.np <= 0) return hipGetLastError();
+  int nexp = 0;
+  const ExpandArgs ea = expand_args(c, target, with_find, &nexp);
+  hipLaunchKernelGGL(k_links_scan, dim3(nsegs), dim3(PB), sizeof(uint32_t) * (size_t)P.seg_cap,
+                     c.stream, scan_args(c, target, overlap_check, istep));
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, c.stream, count_args(c, target));
+  hipLaunchKernelGGL(k_links_expand, dim3(nexp), dim3(PB), 0, c.stream, ea);
+  P.map_np[target] = P.np;
+  return hipGetLastError();
 }
 
 hipError_t launch_bounce_back(Ctx& c, bool fused) {
-  if (c.prec == MFC_PREC_FP64)
-    bb_dispatch<double>(c, fused);
-  else
-    bb_dispatch<float>(c, fused);
+  int nb = 0, ns = 0;
+  const BounceArgs ba = bounce_args(c, fused, &nb);
+  const ScatterArgs sa = scatter_args(c, fused, &ns);
+  if (c.prec == MFC_PREC_FP64) {
+    hipLaunchKernelGGL(k_bounce_back<double>, dim3(nb), dim3(PB), 0, c.stream, ba);
+    hipLaunchKernelGGL(k_bb_scatter<double>, dim3(ns), dim3(PB), 0, c.stream, sa);
+  } else {
+    hipLaunchKernelGGL(k_bounce_back<float>, dim3(nb), dim3(PB), 0, c.stream, ba);
+    hipLaunchKernelGGL(k_bb_scatter<float>, dim3(ns), dim3(PB), 0, c.stream, sa);
+  }
   return hipGetLastError();
 }
 
@@ -1169,17 +1292,9 @@ hipError_t launch_lubforce(Ctx& c) {
 }
 
 hipError_t launch_move(Ctx& c, int first, int wrap) {
-  Particles& P = c.pt;
-  MoveArgs m;
-  m.yp = P.yp; m.ypglb = P.ypglb; m.ypglbp = P.ypglbp;
-  m.wp = P.wp; m.wpp = P.wpp; m.omgp = P.omgp; m.omgpp = P.omgpp;
-  m.fHIp = P.fHIp; m.flubp = P.flubp; m.torqp = P.torqp; m.forcep = P.forcep;
-  m.forcepp = P.forcepp; m.torqpp = P.torqpp; m.dwdt = P.dwdt; m.domgdt = P.domgdt;
-  m.thetap = P.thetap;
-  m.volp = c.p.volp; m.rhog = c.p.rhog; m.amp = c.p.amp; m.aip = c.p.aip;
-  m.bfx = c.p.body_force[0]; m.bfy = c.p.body_force[1]; m.bfz = c.p.body_force[2];
-  hipLaunchKernelGGL(k_move, dim3((P.np + 255) / 256), dim3(256), 0, c.stream, m, P.np, first, wrap,
-                     pgeom(c), c.d_status);
+  int nb = 0;
+  const MoveKArgs a = move_args(c, first, wrap, &nb);
+  hipLaunchKernelGGL(k_move, dim3(nb), dim3(256), 0, c.stream, a);
   return hipGetLastError();
 }
 
@@ -1192,26 +1307,26 @@ hipError_t launch_redistribute(Ctx& c, int istep) {
   return hipGetLastError();
 }
 
+constexpr int kFillBlocks = 1024;
+
 hipError_t launch_filling(Ctx& c, bool find) {
   Particles& P = c.pt;
-  FillArgs fa;
-  if (!make_fill_args(c, P.map_old, P.map_cur, fa)) return hipSuccess;
+  FillApplyArgs a;
+  if (!fill_apply_args(c, kFillBlocks, a)) return hipSuccess;
   if (find)
-    hipLaunchKernelGGL(k_fill_find, dim3(P.map_np[P.map_old] * P.nseg), dim3(PB), 0, c.stream, fa,
+    hipLaunchKernelGGL(k_fill_find, dim3(P.map_np[P.map_old] * P.nseg), dim3(PB), 0, c.stream, a.fa,
                        P.nseg, c.d_status);
-  const dim3 grid(1024), blk(256);
+  const dim3 grid(kFillBlocks), blk(256);
   if (c.prec == MFC_PREC_FP64) {
-    double* A = (double*)c.pop[c.cur];
     if (c.g.walls_y)
-      hipLaunchKernelGGL((k_fill_apply<double, true>), grid, blk, 0, c.stream, A, c.g, fa);
+      hipLaunchKernelGGL((k_fill_apply<double, true>), grid, blk, 0, c.stream, a);
     else
-      hipLaunchKernelGGL((k_fill_apply<double, false>), grid, blk, 0, c.stream, A, c.g, fa);
+      hipLaunchKernelGGL((k_fill_apply<double, false>), grid, blk, 0, c.stream, a);
   } else {
-    float* A = (float*)c.pop[c.cur];
     if (c.g.walls_y)
-      hipLaunchKernelGGL((k_fill_apply<float, true>), grid, blk, 0, c.stream, A, c.g, fa);
+      hipLaunchKernelGGL((k_fill_apply<float, true>), grid, blk, 0, c.stream, a);
     else
-      hipLaunchKernelGGL((k_fill_apply<float, false>), grid, blk, 0, c.stream, A, c.g, fa);
+      hipLaunchKernelGGL((k_fill_apply<float, false>), grid, blk, 0, c.stream, a);
   }
   return hipGetLastError();
 }
@@ -1236,6 +1351,69 @@ hipError_t launch_links_dense(Ctx& c, int32_t* ilink, int32_t* mlink, double* al
   Particles& P = c.pt;
   hipLaunchKernelGGL(k_links_dense, dim3(P.np * P.nseg), dim3(256), 0, c.stream, c.g, P.nseg, P.lk_off,
                      P.lk_cap, P.lk_node, P.lk_dir, P.lk_alpha, ilink, mlink, alink);
+  return hipGetLastError();
+}
+
+}  // namespace mfc
+
+namespace mfc {
+
+size_t stage_arg_bytes(Stage st) {
+  switch (st) {
+    case SG_BOUNCE: return sizeof(BounceArgs);
+    case SG_SCATTER: return sizeof(ScatterArgs);
+    case SG_MOVE: return sizeof(MoveKArgs);
+    case SG_SCAN: return sizeof(ScanArgs);
+    case SG_COUNT: return sizeof(ScanCountArgs);
+    case SG_EXPAND: return sizeof(ExpandArgs);
+    case SG_FILL: return sizeof(FillApplyArgs);
+  }
+  return 0;
+}
+
+hipError_t launch_batched(Stage st, int prec, int walls, const void* t, const int* off, int nc,
+                          int blocks, size_t lds, hipStream_t s) {
+  if (nc <= 0 || blocks <= 0) return hipSuccess;
+  const bool f64 = prec == MFC_PREC_FP64;
+  switch (st) {
+    case SG_BOUNCE:
+      if (f64)
+        hipLaunchKernelGGL(kb_bounce_back<double>, dim3(blocks), dim3(PB), 0, s, (const BounceArgs*)t, off, nc);
+      else
+        hipLaunchKernelGGL(kb_bounce_back<float>, dim3(blocks), dim3(PB), 0, s, (const BounceArgs*)t, off, nc);
+      break;
+    case SG_SCATTER:
+      if (f64)
+        hipLaunchKernelGGL(kb_bb_scatter<double>, dim3(blocks), dim3(PB), 0, s, (const ScatterArgs*)t, off, nc);
+      else
+        hipLaunchKernelGGL(kb_bb_scatter<float>, dim3(blocks), dim3(PB), 0, s, (const ScatterArgs*)t, off, nc);
+      break;
+    case SG_MOVE:
+      hipLaunchKernelGGL(kb_move, dim3(blocks), dim3(256), 0, s, (const MoveKArgs*)t, off, nc);
+      break;
+    case SG_SCAN:
+      hipLaunchKernelGGL(kb_links_scan, dim3(blocks), dim3(PB), lds, s, (const ScanArgs*)t, off, nc);
+      break;
+    case SG_COUNT:
+      hipLaunchKernelGGL(kb_scan_counts, dim3(blocks), dim3(1024), 0, s, (const ScanCountArgs*)t, off, nc);
+      break;
+    case SG_EXPAND:
+      hipLaunchKernelGGL(kb_links_expand, dim3(blocks), dim3(PB), 0, s, (const ExpandArgs*)t, off, nc);
+      break;
+    case SG_FILL:
+      if (f64) {
+        if (walls)
+          hipLaunchKernelGGL((kb_fill_apply<double, true>), dim3(blocks), dim3(256), 0, s, (const FillApplyArgs*)t, off, nc);
+        else
+          hipLaunchKernelGGL((kb_fill_apply<double, false>), dim3(blocks), dim3(256), 0, s, (const FillApplyArgs*)t, off, nc);
+      } else {
+        if (walls)
+          hipLaunchKernelGGL((kb_fill_apply<float, true>), dim3(blocks), dim3(256), 0, s, (const FillApplyArgs*)t, off, nc);
+        else
+          hipLaunchKernelGGL((kb_fill_apply<float, false>), dim3(blocks), dim3(256), 0, s, (const FillApplyArgs*)t, off, nc);
+      }
+      break;
+  }
   return hipGetLastError();
 }
 

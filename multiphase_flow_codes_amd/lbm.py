@@ -98,6 +98,17 @@ SIGNATURES = {
     "mfc_diag_spectrum": [C.c_void_p, C.c_int32, _D],
     "mfc_set_timing": [C.c_void_p, C.c_int32],
     "mfc_get_timing": [C.c_void_p, _D, C.POINTER(C.c_int64)],
+    "mfc_ens_create": [C.POINTER(C.c_void_p)],
+    "mfc_ens_destroy": [C.c_void_p],
+    "mfc_ens_last_error": [C.c_void_p],
+    "mfc_ens_add": [C.c_void_p, C.c_void_p, C.c_int32],
+    "mfc_ens_remove": [C.c_void_p, C.c_void_p],
+    "mfc_ens_size": [C.c_void_p],
+    "mfc_ens_step": [C.c_void_p, C.c_int32],
+    "mfc_ens_sync": [C.c_void_p],
+    "mfc_ens_set_block": [C.c_void_p, C.c_int32],
+    "mfc_ens_autotune": [C.c_void_p, C.c_int32, _I, _D],
+    "mfc_ens_get_stream": [C.c_void_p],
 }
 
 
@@ -115,6 +126,8 @@ def lib():
             fn.restype = C.c_int
         L.mfc_last_error.restype = C.c_char_p
         L.mfc_get_stream.restype = C.c_void_p
+        L.mfc_ens_last_error.restype = C.c_char_p
+        L.mfc_ens_get_stream.restype = C.c_void_p
         _lib = L
     return _lib
 
@@ -317,3 +330,58 @@ class Lattice:
         n = C.c_int64()
         self._chk(lib().mfc_get_timing(self.h, C.byref(t), C.byref(n)))
         return t.value, n.value
+
+
+class Ensemble:
+    """Many boxes advanced together (mfc_ens_*): one grid per step stage for
+    all members.  Members are Lattice objects, ready to step (after macrovar,
+    and beads_links when they hold spheres); they keep their own state."""
+
+    def __init__(self):
+        self.h = C.c_void_p()
+        rc = lib().mfc_ens_create(C.byref(self.h))
+        if rc != 0:
+            raise MfcError(rc, "mfc_ens_create failed")
+        self.members = []
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise MfcError(rc, lib().mfc_ens_last_error(self.h).decode())
+
+    def add(self, lat: "Lattice", istep: int):
+        self._chk(lib().mfc_ens_add(self.h, lat.h, int(istep)))
+        self.members.append(lat)
+
+    def remove(self, lat: "Lattice"):
+        self._chk(lib().mfc_ens_remove(self.h, lat.h))
+        self.members.remove(lat)
+
+    def step(self, nsteps: int = 1):
+        self._chk(lib().mfc_ens_step(self.h, int(nsteps)))
+
+    def sync(self):
+        self._chk(lib().mfc_ens_sync(self.h))
+
+    def set_block(self, bs: int):
+        self._chk(lib().mfc_ens_set_block(self.h, int(bs)))
+
+    def autotune(self, steps: int = 5):
+        """Times collide blocks of 64, 128 and 256 threads on the current
+        members (which advance 3 * steps steps); keeps the fastest."""
+        chosen = C.c_int32()
+        ms = np.zeros(3)
+        self._chk(lib().mfc_ens_autotune(self.h, int(steps), C.byref(chosen), _dp(ms)))
+        return chosen.value, dict(zip((64, 128, 256), ms))
+
+    def close(self):
+        if self.h:
+            lib().mfc_ens_destroy(self.h)
+            self.h = C.c_void_p()
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

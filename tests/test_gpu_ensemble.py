@@ -1,0 +1,94 @@
+"""The ensemble engine (mfc_ens_*): boxes of different sizes, sphere counts,
+radii, densities and body forces advanced together with one grid per step
+stage give bitwise the same populations, particle state and node maps as the
+same boxes advanced one at a time with mfc_step, including a box that joins
+late and one that leaves early; the collide block autotuner keeps results
+unchanged."""
+import numpy as np
+import pytest
+
+from helpers import place_spheres, small_params, smooth_velocity
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # (n, npart, rad, rhopart, body force x, seed)
+    ((24, 20, 28), 2, 3.0, 2.5, 2e-5, 1),
+    ((32, 32, 32), 3, 4.0, 5.0, 5e-6, 2),
+    ((40, 36, 24), 1, 3.5, 1.5, 1e-4, 3),
+    ((28, 28, 28), 0, 3.0, 5.0, 3e-5, 4),
+]
+
+
+def _box(case, precision):
+    from multiphase_flow_codes_amd.lbm import Lattice
+    n, npart, rad, rhop, bf, seed = case
+    prm = small_params(n=n, npart=npart, rad=rad, rhopart=rhop, forcing=False, visc=0.01,
+                       strong_forcing=False, body_force=(bf, 0.0, 0.0))
+    rng = np.random.default_rng(seed)
+    lat = Lattice(prm, precision)
+    ux, uy, uz = smooth_velocity(prm, rng, amp=0.02)
+    lat.upload_macro(None, ux, uy, uz)
+    lat.initpop()
+    lat.rhoupdat()
+    lat.macrovar()
+    if npart:
+        lat.prt_init(npart)
+        lat.prt_set("ypglb", place_spheres(prm, rng, npart, gap=2.5))
+        lat.initpartvel()
+        lat.initpartomg()
+        lat.beads_links()
+    return lat
+
+
+def _same(a, b):
+    assert np.array_equal(a.download_f(), b.download_f())
+    if a.prm.npart:
+        for name in ("ypglb", "wp", "omgp", "fHIp", "torqp", "flubp"):
+            assert np.array_equal(a.prt_get(name), b.prt_get(name)), name
+        for which in (0, 1):
+            ia, sa = a.get_maps(which)
+            ib, sb = b.get_maps(which)
+            assert np.array_equal(ia, ib) and np.array_equal(sa, sb)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_ensemble_equals_boxes_alone(precision):
+    from multiphase_flow_codes_amd.lbm import Ensemble
+    alone = [_box(c, precision) for c in CASES]
+    members = [_box(c, precision) for c in CASES]
+    ens = Ensemble()
+    for lat in members[:3]:
+        ens.add(lat, 1)
+    ens.step(4)
+    ens.add(members[3], 1)            # joins late, at its own step 1
+    ens.step(3)
+    ens.remove(members[0])            # leaves after 7 steps
+    ens.step(2)
+    ens.sync()
+    steps = [7, 9, 9, 5]
+    for lat, n in zip(alone, steps):
+        lat.step(1, n)
+        lat.sync()
+    for a, b in zip(alone, members):
+        _same(a, b)
+    ens.close()
+
+
+def test_autotune_keeps_results():
+    from multiphase_flow_codes_amd.lbm import Ensemble
+    alone = [_box(c, "fp32") for c in CASES]
+    members = [_box(c, "fp32") for c in CASES]
+    ens = Ensemble()
+    for lat in members:
+        ens.add(lat, 1)
+    bs, times = ens.autotune(2)
+    assert bs in (64, 128, 256) and all(t > 0 for t in times.values())
+    ens.step(2)
+    ens.sync()
+    for lat in alone:
+        lat.step(1, 8)
+        lat.sync()
+    for a, b in zip(alone, members):
+        _same(a, b)
+    ens.close()
