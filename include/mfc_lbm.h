@@ -63,6 +63,8 @@ typedef struct mfc_params {
   double stf0, stf1, gscale;              /* para.f90:237-245 */
   double vscale, tscale, xkf, pi2;        /* forcing scales (para.f90:37,60; gaussian.f90:195) */
   double body_force[3];                   /* uniform force added to the forcing field */
+  double particle_force[3];               /* uniform force per unit sphere volume on the spheres
+                                             only (e.g. the buoyancy-corrected weight) */
 } mfc_params;
 
 typedef struct mfc_ctx mfc_ctx;

@@ -703,6 +703,7 @@ __device__ __forceinline__ void move_body(int b_, MoveArgs m, int np, int first,
   if (p >= np) return;
   const double dt = 1.0;
   const double bf[3] = {m.bfx, m.bfy, m.bfz};
+  const double pf[3] = {m.pfx, m.pfy, m.pfz};
   double yp[3], wp[3], omgp[3], fhi[3], flub[3], torq[3], fpp[3], tpp[3], dwdt[3], domgdt[3], th[3];
   double yg[3];
 #pragma unroll
@@ -729,6 +730,7 @@ __device__ __forceinline__ void move_body(int b_, MoveArgs m, int np, int first,
     double F = fhi[c] + flub[c];
     if (c == 1) F = F - m.volp * m.rhog;
     if (bf[c] != 0.0) F = F + m.volp * bf[c];
+    if (pf[c] != 0.0) F = F + m.volp * pf[c];
     fp[c] = F;
     if (first) fpp[c] = F;
     dwn[c] = 0.5 * (F + fpp[c]) / m.amp;
@@ -1240,6 +1242,7 @@ MoveKArgs move_args(Ctx& c, int first, int wrap, int* blocks) {
   m.thetap = P.thetap;
   m.volp = c.p.volp; m.rhog = c.p.rhog; m.amp = c.p.amp; m.aip = c.p.aip;
   m.bfx = c.p.body_force[0]; m.bfy = c.p.body_force[1]; m.bfz = c.p.body_force[2];
+  m.pfx = c.p.particle_force[0]; m.pfy = c.p.particle_force[1]; m.pfz = c.p.particle_force[2];
   a.np = P.np; a.first = first; a.wrap = wrap; a.G = pgeom(c); a.status = c.d_status;
   *blocks = (P.np + 255) / 256;
   return a;

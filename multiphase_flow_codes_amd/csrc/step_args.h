@@ -58,7 +58,8 @@ struct MoveArgs {
   double *yp, *ypglb, *ypglbp, *wp, *wpp, *omgp, *omgpp, *fHIp, *flubp, *torqp, *forcep, *forcepp,
       *torqpp, *dwdt, *domgdt, *thetap;
   double volp, rhog, amp, aip;
-  double bfx, bfy, bfz;
+  double bfx, bfy, bfz;   // fluid body force (acts on the displaced volume)
+  double pfx, pfy, pfz;   // force per unit sphere volume on the spheres only
 };
 
 struct ScanArgs {

@@ -114,6 +114,7 @@
       p%xkf = sqrt(8.0d0)
       p%pi2 = pi2
       p%body_force = 0.0d0
+      p%particle_force = 0.0d0
       rc = mfc_create(p, ctx)
       if (rc /= MFC_OK) then
         write(*,*) 'fault: mfc_create failed (', rc, ')'

@@ -26,6 +26,7 @@
         real(c_double) :: stf0, stf1, gscale
         real(c_double) :: vscale, tscale, xkf, pi2
         real(c_double) :: body_force(3)
+        real(c_double) :: particle_force(3)
       end type mfc_params
 
       interface

@@ -50,6 +50,7 @@ class MfcParams(C.Structure):
         ("stf0", C.c_double), ("stf1", C.c_double), ("gscale", C.c_double),
         ("vscale", C.c_double), ("tscale", C.c_double), ("xkf", C.c_double), ("pi2", C.c_double),
         ("body_force", C.c_double * 3),
+        ("particle_force", C.c_double * 3),
     ]
 
 
@@ -147,6 +148,7 @@ def make_params(prm: LBMParams, precision: str = "fp64", npart_max: int | None =
     p.xkf = float(prm.xkf) if prm.forcing else 0.0
     for k in range(3):
         p.body_force[k] = float(prm.body_force[k])
+        p.particle_force[k] = float(prm.particle_force[k])
     return p
 
 

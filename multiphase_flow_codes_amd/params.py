@@ -55,6 +55,8 @@ class LBMParams:
     iseedf: int = ISEEDF_REF
     # uniform body force added to the forcing field (fluid and particles)
     body_force: tuple = (0.0, 0.0, 0.0)
+    # uniform force per unit sphere volume on the spheres only
+    particle_force: tuple = (0.0, 0.0, 0.0)
     # half-way bounce-back walls bounding the y direction
     walls_y: bool = False
     # derived (para.f90)

@@ -296,7 +296,9 @@ void orc_collision_mrt(const orc_params *P, orc_state *S) {
     if (S->ibnodes[n] >= 0) continue;
     r = S->rho[n]; u = S->ux[n]; v = S->uy[n]; w = S->uz[n];
     us = u * u; vs = v * v; ws = w * w;
-    Fx = S->fx[n]; Fy = S->fy[n]; Fz = S->fz[n];
+    Fx = S->fx[n] + P->body_force[0];
+    Fy = S->fy[n] + P->body_force[1];
+    Fz = S->fz[n] + P->body_force[2];
     G3 = u * Fx + v * Fy + w * Fz;
     Fb[0] = -G3;
     for (ip = 1; ip < NPOP; ++ip) {
@@ -481,9 +483,9 @@ void orc_macrovar(const orc_params *P, orc_state *S) {
     double w = g[5] - g[6] + a3 - a4 + a5 - a6;
     double r = g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6] + g[7] + g[8] + g[9] + g[10] +
                g[11] + g[12] + g[13] + g[14] + g[15] + g[16] + g[17] + g[18];
-    S->ux[n] = u + S->fx[n] / 2.0;
-    S->uy[n] = v + S->fy[n] / 2.0;
-    S->uz[n] = w + S->fz[n] / 2.0;
+    S->ux[n] = u + (S->fx[n] + P->body_force[0]) / 2.0;
+    S->uy[n] = v + (S->fy[n] + P->body_force[1]) / 2.0;
+    S->uz[n] = w + (S->fz[n] + P->body_force[2]) / 2.0;
     S->rho[n] = r;
   }
   if (!P->ipart) return;
@@ -944,6 +946,13 @@ void orc_beads_move(const orc_params *P, orc_state *S) {
     double dwn[3], don[3];
     for (c = 0; c < 3; ++c) S->forcep[3 * p + c] = S->fHIp[3 * p + c] + S->flubp[3 * p + c];
     S->forcep[3 * p + 1] = S->forcep[3 * p + 1] - P->volp * P->rhog;
+    /* uniform forces: the fluid's body force acts on the displaced volume, and
+       the spheres' own force (e.g. buoyancy-corrected weight) */
+    for (c = 0; c < 3; ++c) {
+      if (P->body_force[c] != 0.0) S->forcep[3 * p + c] = S->forcep[3 * p + c] + P->volp * P->body_force[c];
+      if (P->particle_force[c] != 0.0)
+        S->forcep[3 * p + c] = S->forcep[3 * p + c] + P->volp * P->particle_force[c];
+    }
     for (c = 0; c < 3; ++c) {
       int q = 3 * p + c;
       if (first) S->forcepp[q] = S->forcep[q];

@@ -38,6 +38,9 @@ typedef struct orc_params {
   double ww0, ww1, ww2;
   double volp, amp, aip, rhog, stf0, stf1, gscale;
   double vscale, tscale, xkf, pi2;
+  double body_force[3];      /* uniform force density on the fluid, added to the forcing
+                                field in collision and macrovar, and V_p times it on spheres */
+  double particle_force[3];  /* uniform force per unit sphere volume on the spheres only */
 } orc_params;
 
 typedef struct orc_state {

@@ -30,6 +30,7 @@ class OrcParams(C.Structure):
         ("volp", C.c_double), ("amp", C.c_double), ("aip", C.c_double), ("rhog", C.c_double),
         ("stf0", C.c_double), ("stf1", C.c_double), ("gscale", C.c_double),
         ("vscale", C.c_double), ("tscale", C.c_double), ("xkf", C.c_double), ("pi2", C.c_double),
+        ("body_force", C.c_double * 3), ("particle_force", C.c_double * 3),
     ]
 
 
@@ -110,7 +111,11 @@ class Oracle:
         npart = prm.npart
         self.P = OrcParams()
         for name, _ in OrcParams._fields_:
-            if hasattr(prm, name):
+            if name in ("body_force", "particle_force"):
+                v = getattr(prm, name, (0.0, 0.0, 0.0))
+                for k in range(3):
+                    getattr(self.P, name)[k] = float(v[k])
+            elif hasattr(prm, name):
                 setattr(self.P, name, getattr(prm, name))
         self.P.ipart = int(npart > 0 if ipart is None else ipart)
         self.P.bb_mode = bb_mode

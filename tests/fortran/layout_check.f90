@@ -46,6 +46,7 @@
       call show('xkf', c_loc(p%xkf))
       call show('pi2', c_loc(p%pi2))
       call show('body_force', c_loc(p%body_force))
+      call show('particle_force', c_loc(p%particle_force))
       contains
       subroutine show(name, a)
         character(len=*), intent(in) :: name

@@ -202,3 +202,19 @@ def test_multistep_call_equals_fine_grained(oracle_lib):
         ib, sb = b.get_maps(which)
         assert np.array_equal(ia, ib) and np.array_equal(sa, sb)
     assert a.status()["refill"] == b.status()["refill"]
+
+
+def test_body_and_sphere_forces_match_oracle(oracle_lib):
+    """Sedimentation drive: sphere-only force plus the fluid's counter body
+    force (no forcing modes), spheres released from rest."""
+    rng = np.random.default_rng(12)
+    prm = small_params(n=(32, 32, 32), npart=3, rad=4.0, rhopart=2.0, forcing=False,
+                       strong_forcing=False, body_force=(-4e-6, 0.0, 1e-6),
+                       particle_force=(6e-5, 0.0, -2e-5))
+    lat, orc = _run_particle_case(prm, 8, rng, fused=False)
+    assert rel(lat.download_f(), orc.f) < TOL64
+    for name in ("ypglb", "wp", "omgp", "fHIp", "torqp"):
+        assert rel(lat.prt_get(name), getattr(orc, name)) < TOL64, name
+    rho, ux, uy, uz = lat.download_macro()
+    for a, b in ((rho, orc.rho), (ux, orc.ux), (uy, orc.uy), (uz, orc.uz)):
+        assert rel(a, b) < TOL64

@@ -11,7 +11,9 @@ bit, see test_oracle_ref.py):
   orc_initpop/feqpnt equilibrium moments (density, momentum, momentum flux)
   orc_beads_links    brute-force solid set and link set; wall point on the sphere
   orc_beads_collision  rest/uniform-density force zero; moving-sphere force closed form
-  orc_beads_move     constant force/torque: uniformly accelerated motion, exact
+  orc_beads_move     constant force/torque, body and sphere forces: uniformly
+                     accelerated motion, exact
+  body force         uniform fluid acceleration n F after n steps (collision + macrovar)
   orc_beads_lubforce Newton's third law, line of centres, cut-off continuity
   orc_beads_filling  uniform equilibrium fluid: refill = feq(rho, rigid-body u)
   orc_initpartvel    trilinear interpolation exact on a linear field
@@ -339,6 +341,43 @@ def test_move_uniform_acceleration_exact(oracle_lib):
         assert np.allclose(o.omgp[0], n * Tc / prm.aip, rtol=0, atol=1e-16)
         assert np.allclose(o.yp[0], y0 + 0.5 * Fc / prm.amp * n * n, rtol=0, atol=1e-13)
         assert np.allclose(o.thetap[0], 0.5 * Tc / prm.aip * n * n, rtol=0, atol=1e-15)
+
+
+def test_move_uniform_forces_exact(oracle_lib):
+    """Body force (on the displaced volume) and sphere-only force: a free
+    sphere with no fluid force accelerates at V_p (F_body + F_sphere) / m_p."""
+    bf, pf = np.array([2e-4, 0.0, -1e-4]), np.array([0.0, 3e-4, 5e-4])
+    prm = LBMParams(nx=32, ny=32, nz=32, visc=0.01, npart=1, rad=3.0, body_force=tuple(bf),
+                    particle_force=tuple(pf))
+    o = _orc(prm)
+    a = prm.volp * (bf + pf) / prm.amp
+    y0 = np.array([10.0, 11.0, 12.0])
+    o.yp[0] = y0
+    for n in range(1, 7):
+        o.S.istep = n
+        o.fHIp[0] = 0.0
+        o.torqp[0] = 0.0
+        o.flubp[0] = 0.0
+        o.beads_move()
+        assert np.allclose(o.wp[0], n * a, rtol=0, atol=1e-18)
+        assert np.allclose(o.yp[0], y0 + 0.5 * a * n * n, rtol=0, atol=1e-13)
+
+
+def test_body_force_accelerates_fluid_uniformly(oracle_lib):
+    """No spheres, no forcing modes, uniform body force F from rest: the
+    macroscopic velocity after n steps is exactly n F at every node (the
+    momentum gains F per step; macrovar adds F/2), density unchanged."""
+    F = np.array([3e-6, -2e-6, 1e-6])
+    prm = LBMParams(nx=8, ny=6, nz=10, visc=0.02, forcing=False, body_force=tuple(F))
+    o = _orc(prm)
+    o.initpop()
+    o.rhoupdat()
+    for n in range(1, 6):
+        o.S.istep = n
+        o.call("timestep")
+        for comp, Fc in zip((o.ux, o.uy, o.uz), F):
+            assert np.allclose(comp, n * Fc, rtol=1e-12, atol=1e-20)
+        assert np.max(np.abs(o.rho)) < 1e-16
 
 
 def test_lubrication_pair_properties(oracle_lib):
