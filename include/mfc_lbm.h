@@ -65,6 +65,9 @@ typedef struct mfc_params {
   double body_force[3];                   /* uniform force added to the forcing field */
   double particle_force[3];               /* uniform force per unit sphere volume on the spheres
                                              only (e.g. the buoyancy-corrected weight) */
+  double mingap_w, stf0_w, stf1_w;        /* walls_y: sphere-wall lubrication range (also the
+                                             placement clearance) and stiffness
+                                             (3DChannel_.../var_inc.f90:31, para.f90:241-242) */
 } mfc_params;
 
 typedef struct mfc_ctx mfc_ctx;

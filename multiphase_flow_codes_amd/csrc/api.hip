@@ -781,7 +781,15 @@ int mfc_initpartpos(mfc_ctx* ctx, int32_t* iseedp, int32_t mingap_check) {
   for (int round = 0;; ++round) {
     for (int p = 0; p < np; ++p)
       if (flag[p] > 0)
-        for (int k = 0; k < 3; ++k) y[3 * p + k] = (double)ns[k] * ran1(&idum, &iy, iv);
+        for (int k = 0; k < 3; ++k) {
+          if (k == 1 && c.g.walls_y) {
+            // clear of the walls by rad + mingap_w (channel initpartpos, wall axis y)
+            const double rg = c.p.rad + c.p.mingap_w;
+            y[3 * p + 1] = rg + ((double)c.g.ny - 2.0 * rg) * ran1(&idum, &iy, iv);
+          } else {
+            y[3 * p + k] = (double)ns[k] * ran1(&idum, &iy, iv);
+          }
+        }
     int cnt = 0;
     for (int a = 0; a < np; ++a) {
       flag[a] = 0;

@@ -513,6 +513,23 @@ __device__ void lub_block(int a, const double* __restrict__ ypglb, int np, const
     double s = 0.0;
 #pragma unroll
     for (int q = 0; q < PB / 64; ++q) s += red[threadIdx.x][q];
+    if (G.walls_y && threadIdx.x == 1) {
+      // sphere-wall lubrication (channel beads_lubforce with its wall axis
+      // moved to y): the wall at y = 0 pushes up, the wall at y = ny down
+      const double dist0 = L.mingap_w + G.rad;
+      const double yd = ypglb[3 * a + 1];
+      double dist = yd, hgap = dist - G.rad;
+      if (dist < dist0) {
+        const double d1 = hgap < 0.0 ? -hgap : 0.0, t = (dist0 - dist) / L.mingap_w;
+        s += (t * t / L.stf0_w + d1 / L.mingap_w / L.stf1_w) * L.fpc;
+      }
+      dist = (double)G.ny - yd;
+      hgap = dist - G.rad;
+      if (dist < dist0) {
+        const double d1 = hgap < 0.0 ? -hgap : 0.0, t = (dist0 - dist) / L.mingap_w;
+        s -= (t * t / L.stf0_w + d1 / L.mingap_w / L.stf1_w) * L.fpc;
+      }
+    }
     L.flubp[3 * a + threadIdx.x] = s;
   }
 }
@@ -1162,6 +1179,9 @@ static LubArgs lub_args(Ctx& c) {
   L.fpc = c.p.volp * c.p.rho0 * 9.8 / c.p.gscale;
   L.stf0 = c.p.stf0;
   L.stf1 = c.p.stf1;
+  L.mingap_w = c.p.mingap_w;
+  L.stf0_w = c.p.stf0_w;
+  L.stf1_w = c.p.stf1_w;
   L.flubp = c.pt.flubp;
   return L;
 }

@@ -51,6 +51,7 @@ struct FillArgs {
 
 struct LubArgs {
   double mingap, fpc, stf0, stf1;
+  double mingap_w, stf0_w, stf1_w;   // sphere-wall lubrication (walls_y)
   double* flubp;
 };
 

@@ -115,6 +115,10 @@
       p%pi2 = pi2
       p%body_force = 0.0d0
       p%particle_force = 0.0d0
+      ! the periodic box has no walls; the channel's defaults
+      p%mingap_w = mingap
+      p%stf0_w = stf0
+      p%stf1_w = stf1
       rc = mfc_create(p, ctx)
       if (rc /= MFC_OK) then
         write(*,*) 'fault: mfc_create failed (', rc, ')'

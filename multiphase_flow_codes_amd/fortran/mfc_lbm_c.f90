@@ -27,6 +27,7 @@
         real(c_double) :: vscale, tscale, xkf, pi2
         real(c_double) :: body_force(3)
         real(c_double) :: particle_force(3)
+        real(c_double) :: mingap_w, stf0_w, stf1_w
       end type mfc_params
 
       interface

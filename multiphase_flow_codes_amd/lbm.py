@@ -51,6 +51,7 @@ class MfcParams(C.Structure):
         ("vscale", C.c_double), ("tscale", C.c_double), ("xkf", C.c_double), ("pi2", C.c_double),
         ("body_force", C.c_double * 3),
         ("particle_force", C.c_double * 3),
+        ("mingap_w", C.c_double), ("stf0_w", C.c_double), ("stf1_w", C.c_double),
     ]
 
 
@@ -143,7 +144,8 @@ def make_params(prm: LBMParams, precision: str = "fp64", npart_max: int | None =
     p.irelease = prm.irelease
     for name in ["s1", "s2", "s4", "s9", "s10", "s13", "s16", "omegepsl", "omegepslj", "omegxx",
                  "ww0", "ww1", "ww2", "rho0", "rad", "rhopart", "mingap", "volp", "amp", "aip",
-                 "rhog", "stf0", "stf1", "gscale", "vscale", "tscale", "pi2"]:
+                 "rhog", "stf0", "stf1", "gscale", "vscale", "tscale", "pi2", "mingap_w", "stf0_w",
+                 "stf1_w"]:
         setattr(p, name, float(getattr(prm, name)))
     p.xkf = float(prm.xkf) if prm.forcing else 0.0
     for k in range(3):

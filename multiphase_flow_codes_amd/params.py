@@ -59,6 +59,11 @@ class LBMParams:
     particle_force: tuple = (0.0, 0.0, 0.0)
     # half-way bounce-back walls bounding the y direction
     walls_y: bool = False
+    # sphere-wall lubrication range (also the spheres' placement clearance from
+    # the walls) and stiffness (3DChannel_.../var_inc.f90:31, para.f90:241-242)
+    mingap_w: float = 2.0
+    stf0_w: float = 0.025
+    stf1_w: float = 0.002
     # derived (para.f90)
     tau: float = field(default=0.0, init=False)
     pi2: float = field(default=0.0, init=False)

@@ -470,6 +470,11 @@ static inline double near_img(double c, double pnt, int n) {
   return c;
 }
 
+/* y image of a centre: none across walls */
+static inline double y_img(const orc_params *P, double c, double pnt) {
+  return P->walls_y ? c : near_img(c, pnt, P->ny);
+}
+
 /* macrovar (BOX/collision.f90:232-332) */
 void orc_macrovar(const orc_params *P, orc_state *S) {
   long n, nt = ncells(P);
@@ -504,7 +509,7 @@ void orc_macrovar(const orc_params *P, orc_state *S) {
           yp = (double)(j + 1) - 0.5;
           zp = (double)(k + 1) - 0.5;
           xc = near_img(S->ypglb[3 * id], xp, P->nx);
-          yc = near_img(S->ypglb[3 * id + 1], yp, P->ny);
+          yc = y_img(P, S->ypglb[3 * id + 1], yp);
           zc = near_img(S->ypglb[3 * id + 2], zp, P->nz);
           xx = xp - xc; yy = yp - yc; zz = zp - zc;
           W = S->wp + 3 * id; O = S->omgp + 3 * id;
@@ -526,7 +531,7 @@ int orc_ovlpchk(const orc_params *P, orc_state *S, int *flags) {
     double xc = S->yp[3 * a], yc = S->yp[3 * a + 1], zc = S->yp[3 * a + 2];
     for (b = a + 1; b < P->npart; ++b) {
       double xb = near_img(S->ypglb[3 * b], xc, P->nx);
-      double yb = near_img(S->ypglb[3 * b + 1], yc, P->ny);
+      double yb = y_img(P, S->ypglb[3 * b + 1], yc);
       double zb = near_img(S->ypglb[3 * b + 2], zc, P->nz);
       double dist = sqrt((xc - xb) * (xc - xb) + (yc - yb) * (yc - yb) + (zc - zb) * (zc - zb));
       double hgap = dist - 2.0 * P->rad;
@@ -552,7 +557,13 @@ int orc_initpartpos(const orc_params *P, orc_state *S) {
     for (p = 0; p < P->npart; ++p)
       if (flag[p] > 0) {
         S->ypglb[3 * p] = (double)P->nx * orc_ranf(&S->iseedp, &S->iyp, S->ivp);
-        S->ypglb[3 * p + 1] = (double)P->ny * orc_ranf(&S->iseedp, &S->iyp, S->ivp);
+        if (P->walls_y) {
+          /* clear of the walls by rad + mingap_w (channel initpartpos, wall axis y) */
+          const double rg = P->rad + P->mingap_w;
+          S->ypglb[3 * p + 1] = rg + ((double)P->ny - 2.0 * rg) * orc_ranf(&S->iseedp, &S->iyp, S->ivp);
+        } else {
+          S->ypglb[3 * p + 1] = (double)P->ny * orc_ranf(&S->iseedp, &S->iyp, S->ivp);
+        }
         S->ypglb[3 * p + 2] = (double)P->nz * orc_ranf(&S->iseedp, &S->iyp, S->ivp);
       }
     for (p = 0; p < 3 * P->npart; ++p) S->yp[p] = S->ypglb[p];
@@ -626,8 +637,10 @@ void orc_beads_links(const orc_params *P, orc_state *S) {
       if (!(fabs(zz0) <= radp2)) continue;
       for (j = 1; j <= P->ny; ++j) {
         double yp = (double)j - 0.5, yy0;
-        if ((yc - yp) > hy) yc = yc - (double)P->ny;
-        if ((yc - yp) < -hy) yc = yc + (double)P->ny;
+        if (!P->walls_y) {
+          if ((yc - yp) > hy) yc = yc - (double)P->ny;
+          if ((yc - yp) < -hy) yc = yc + (double)P->ny;
+        }
         yy0 = yp - yc;
         if (!(fabs(yy0) <= radp2)) continue;
         for (i = 1; i <= P->nx; ++i) {
@@ -808,7 +821,7 @@ void orc_beads_collision(const orc_params *P, orc_state *S) {
           yp = (double)j - 0.5;
           zp = (double)k - 0.5;
           xc = near_img(S->ypglb[3 * id], xp, nx);
-          yc = near_img(S->ypglb[3 * id + 1], yp, ny);
+          yc = y_img(P, S->ypglb[3 * id + 1], yp);
           zc = near_img(S->ypglb[3 * id + 2], zp, nz);
           W = S->wp + 3 * id;
           O = S->omgp + 3 * id;
@@ -840,9 +853,9 @@ void orc_beads_collision(const orc_params *P, orc_state *S) {
 #define IB(a, b, c) (S->ibnodes[nid(P, (a) - 1, wrap((b) - 1, ny), wrap((c) - 1, nz))])
           if (alpha <= 0.5) {
             ff2 = RD(ip, im1, jm1, km1);
-            ibm1 = IB(im1, jm1, km1);
+            ibm1 = (P->walls_y && (jm1 < 1 || jm1 > ny)) ? 2 : IB(im1, jm1, km1);
             ff3 = RD(ip, im2, jm2, km2);
-            ibm2 = IB(im2, jm2, km2);
+            ibm2 = (P->walls_y && (jm2 < 1 || jm2 > ny)) ? 2 : IB(im2, jm2, km2);
             if (ibm1 > 0) {
               f9 = ff1 - 6.0 * wt * uwpro;
             } else if (ibm2 > 0) {
@@ -856,7 +869,7 @@ void orc_beads_collision(const orc_params *P, orc_state *S) {
           } else {
             ff2 = (P->bb_mode == 0) ? *bb_slot(&V, ipp, i, j, k, NULL) : fsnap[ipp + 19 * m];
             ff3 = RD(ipp, im1, jm1, km1);
-            ibm1 = IB(im1, jm1, km1);
+            ibm1 = (P->walls_y && (jm1 < 1 || jm1 > ny)) ? 2 : IB(im1, jm1, km1);
             if (ibm1 > 0) {
               c1 = 0.5 / alpha;
               c2 = 1.0 - c1;
@@ -870,7 +883,9 @@ void orc_beads_collision(const orc_params *P, orc_state *S) {
           }
 #undef RD
 #undef IB
-          if (P->bb_mode == 0) {
+          if (P->walls_y && (jp1 < 1 || jp1 > ny)) {
+            /* the solid neighbour lies beyond a wall: nothing to write */
+          } else if (P->bb_mode == 0) {
             unsigned char *fl = NULL;
             double *dst = bb_slot(&V, ipp, ip1, jp1, kp1, &fl);
             *dst = f9;
@@ -907,7 +922,7 @@ void orc_beads_lubforce(const orc_params *P, orc_state *S) {
     double xc = S->yp[3 * a], yc = S->yp[3 * a + 1], zc = S->yp[3 * a + 2];
     for (b = a + 1; b < P->npart; ++b) {
       double xb = near_img(S->ypglb[3 * b], xc, P->nx);
-      double yb = near_img(S->ypglb[3 * b + 1], yc, P->ny);
+      double yb = y_img(P, S->ypglb[3 * b + 1], yc);
       double zb = near_img(S->ypglb[3 * b + 2], zc, P->nz);
       double dx = xc - xb, dy = yc - yb, dz = zc - zb;
       double dist = sqrt(dx * dx + dy * dy + dz * dz);
@@ -925,6 +940,28 @@ void orc_beads_lubforce(const orc_params *P, orc_state *S) {
       S->flubp[3 * b] -= fpd * dx;
       S->flubp[3 * b + 1] -= fpd * dy;
       S->flubp[3 * b + 2] -= fpd * dz;
+    }
+  }
+  if (P->walls_y) {
+    /* sphere-wall lubrication (channel beads_lubforce, wall axis y): the
+       bottom wall at y = 0 pushes up, the top wall at y = ny down */
+    const double dist0 = P->mingap_w + P->rad;
+    const double fpc = P->volp * P->rho0 * 9.8 / P->gscale;
+    for (a = 0; a < P->npart; ++a) {
+      const double yd = S->yp[3 * a + 1];
+      double dist = yd, hgap = dist - P->rad, d1, t;
+      if (dist < dist0) {
+        d1 = hgap < 0.0 ? -hgap : 0.0;
+        t = (dist0 - dist) / P->mingap_w;
+        S->flubp[3 * a + 1] += (t * t / P->stf0_w + d1 / P->mingap_w / P->stf1_w) * fpc;
+      }
+      dist = (double)P->ny - yd;
+      hgap = dist - P->rad;
+      if (dist < dist0) {
+        d1 = hgap < 0.0 ? -hgap : 0.0;
+        t = (dist0 - dist) / P->mingap_w;
+        S->flubp[3 * a + 1] -= (t * t / P->stf0_w + d1 / P->mingap_w / P->stf1_w) * fpc;
+      }
     }
   }
 }
@@ -997,8 +1034,10 @@ int orc_beads_redistribute(const orc_params *P, orc_state *S) {
     double x0 = y[0], y0 = y[1], z0 = y[2];
     if (x0 >= (double)P->nx) y[0] = y[0] - (double)P->nx;
     if (x0 < 0.0) y[0] = y[0] + (double)P->nx;
-    if (y0 >= (double)P->ny) y[1] = y[1] - (double)P->ny;
-    if (y0 < 0.0) y[1] = y[1] + (double)P->ny;
+    if (!P->walls_y) {
+      if (y0 >= (double)P->ny) y[1] = y[1] - (double)P->ny;
+      if (y0 < 0.0) y[1] = y[1] + (double)P->ny;
+    }
     if (z0 >= (double)P->nz) y[2] = y[2] - (double)P->nz;
     if (z0 < 0.0) y[2] = y[2] + (double)P->nz;
   }
@@ -1043,7 +1082,7 @@ void orc_beads_filling(const orc_params *P, orc_state *S) {
         yp = (double)(j + 1) - 0.5;
         zp = (double)(k + 1) - 0.5;
         xc = near_img(S->ypglbp[3 * id] * 1.0, xp, nx);
-        yc = near_img(S->ypglbp[3 * id + 1] * 1.0, yp, ny);
+        yc = y_img(P, S->ypglbp[3 * id + 1] * 1.0, yp);
         zc = near_img(S->ypglbp[3 * id + 2] * 1.0, zp, nz);
         w1 = -0.5 * (S->wp[3 * id] * 1.0 + S->wpp[3 * id] * 1.0);
         w2 = -0.5 * (S->wp[3 * id + 1] * 1.0 + S->wpp[3 * id + 1] * 1.0);
@@ -1081,7 +1120,8 @@ void orc_beads_filling(const orc_params *P, orc_state *S) {
         {
           int a = wrap(i + CX[ipmx], nx), b = wrap(j + CY[ipmx], ny), c = wrap(k + CZ[ipmx], nz);
           long q = nid(P, a, b, c);
-          if (S->ibnodes[q] < 0 && S->ibnodes0[q] < 0) {
+          const int beyond = P->walls_y && (j + CY[ipmx] < 0 || j + CY[ipmx] >= ny);
+          if (!beyond && S->ibnodes[q] < 0 && S->ibnodes0[q] < 0) {
             rho9 = 0.0; u9 = 0.0; v9 = 0.0; w9 = 0.0;
             g = S->f + 19 * q;
             for (ip = 0; ip < NPOP; ++ip) {
@@ -1101,6 +1141,7 @@ void orc_beads_filling(const orc_params *P, orc_state *S) {
         for (ip = 1; ip < NPOP; ++ip) {
           int a = wrap(i + CX[ip], nx), b = wrap(j + CY[ip], ny), c = wrap(k + CZ[ip], nz);
           long q = nid(P, a, b, c);
+          if (P->walls_y && (j + CY[ip] < 0 || j + CY[ip] >= ny)) continue;
           if (S->ibnodes[q] < 0 && S->ibnodes0[q] < 0) {
             int ii;
             nghb++;

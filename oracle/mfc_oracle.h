@@ -15,7 +15,13 @@
  *   - per-node scalar fields (rho, ux, ..., ibnodes) use i + nx*(j + ny*k);
  *   - link arrays ilink/mlink/alink(1:18,lx,ly,lz) use (ip-1) + 18*node;
  *   - particle arrays (3,npart) use c + 3*p (p 0-based, ids in maps 1-based);
- *   - one rank, periodic in all three directions.
+ *   - one rank, periodic in x and z; y periodic too unless walls_y, in which
+ *     case half-way bounce-back walls bound y (node j sits at j + 1/2 between
+ *     walls at 0 and ny), centres are never imaged across them, links and
+ *     interpolation stencils beyond them count as solid, and spheres feel the
+ *     walls' lubrication force (the channel code's wall semantics,
+ *     3DChannel_Decaying_HIT_Finite_Size_Moving_Particles/partlib.f90, with
+ *     its wall axis moved from x to y).
  */
 #ifndef MFC_ORACLE_H
 #define MFC_ORACLE_H
@@ -41,6 +47,8 @@ typedef struct orc_params {
   double body_force[3];      /* uniform force density on the fluid, added to the forcing
                                 field in collision and macrovar, and V_p times it on spheres */
   double particle_force[3];  /* uniform force per unit sphere volume on the spheres only */
+  double mingap_w, stf0_w, stf1_w;  /* sphere-wall lubrication and placement clearance
+                                       (3DChannel_.../var_inc.f90:31, para.f90:241-242) */
 } orc_params;
 
 typedef struct orc_state {

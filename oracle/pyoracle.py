@@ -31,6 +31,7 @@ class OrcParams(C.Structure):
         ("stf0", C.c_double), ("stf1", C.c_double), ("gscale", C.c_double),
         ("vscale", C.c_double), ("tscale", C.c_double), ("xkf", C.c_double), ("pi2", C.c_double),
         ("body_force", C.c_double * 3), ("particle_force", C.c_double * 3),
+        ("mingap_w", C.c_double), ("stf0_w", C.c_double), ("stf1_w", C.c_double),
     ]
 
 

@@ -47,6 +47,9 @@
       call show('pi2', c_loc(p%pi2))
       call show('body_force', c_loc(p%body_force))
       call show('particle_force', c_loc(p%particle_force))
+      call show('mingap_w', c_loc(p%mingap_w))
+      call show('stf0_w', c_loc(p%stf0_w))
+      call show('stf1_w', c_loc(p%stf1_w))
       contains
       subroutine show(name, a)
         character(len=*), intent(in) :: name

@@ -218,3 +218,57 @@ def test_body_and_sphere_forces_match_oracle(oracle_lib):
     rho, ux, uy, uz = lat.download_macro()
     for a, b in ((rho, orc.rho), (ux, orc.ux), (uy, orc.uy), (uz, orc.uz)):
         assert rel(a, b) < TOL64
+
+
+def test_channel_walls_with_spheres_match_oracle(oracle_lib):
+    """Walls on the y faces: a sphere inside the wall-lubrication range, one
+    whose bounding box crosses a wall, links and stencils beyond the walls,
+    body-force drive; maps bit-exact, state to 1e-12."""
+    from multiphase_flow_codes_amd.lbm import Lattice
+    from oracle.pyoracle import Oracle
+    rng = np.random.default_rng(21)
+    prm = small_params(n=(32, 28, 32), npart=3, rad=4.0, rhopart=2.0, walls=True, forcing=False,
+                       strong_forcing=False, body_force=(2e-5, 0.0, 0.0), mingap_w=3.0)
+    lat = Lattice(prm)
+    lat.prt_init(prm.npart)
+    orc = Oracle(prm, bb_mode=1)
+    start_both(prm, lat, orc, rng, amp=0.02)
+    orc.macrovar()
+    lat.macrovar()
+    pos = np.array([[6.3, 5.2, 7.7], [20.1, 14.4, 18.9], [11.8, 22.6, 27.3]])
+    orc.ypglb[:] = pos
+    orc.yp[:] = pos
+    orc.S.istep00 = 1
+    orc.call("initpartvel")
+    orc.call("initpartomg")
+    orc.beads_links()
+    lat.prt_set("ypglb", pos)
+    lat.initpartvel()
+    lat.initpartomg()
+    lat.beads_links()
+    ib, isn = lat.get_maps(0)
+    assert np.array_equal(ib, orc.ibnodes) and np.array_equal(isn, orc.isnodes)
+    for it in range(1, 9):
+        orc.S.istep = it
+        orc.call("timestep")
+        lat.step(it, 1)
+    lat.sync()
+    assert orc.S.nconflict == 0
+    assert np.any(orc.flubp[:, 1] != 0.0)          # the wall lubrication acted
+    assert rel(lat.download_f(), orc.f) < TOL64
+    for name in ("ypglb", "wp", "omgp", "fHIp", "torqp", "flubp"):
+        assert rel(lat.prt_get(name), getattr(orc, name)) < TOL64, name
+    ib, isn = lat.get_maps(0)
+    assert np.array_equal(ib, orc.ibnodes) and np.array_equal(isn, orc.isnodes)
+    il, ml, al = lat.get_links()
+    assert np.array_equal(il, orc.ilink) and np.array_equal(ml, orc.mlink)
+
+
+def test_channel_initpartpos_matches_oracle(oracle_lib):
+    prm = small_params(n=(40, 40, 40), npart=10, rad=3.0, walls=True, mingap_w=4.0)
+    lat, orc = _pair(prm)
+    orc.S.istep00 = 0
+    orc.initpartpos()
+    seed_out = lat.initpartpos(prm.iseedp, mingap_check=True)
+    assert np.array_equal(lat.prt_get("ypglb"), orc.ypglb)
+    assert seed_out == orc.S.iseedp

@@ -14,7 +14,10 @@ bit, see test_oracle_ref.py):
   orc_beads_move     constant force/torque, body and sphere forces: uniformly
                      accelerated motion, exact
   body force         uniform fluid acceleration n F after n steps (collision + macrovar)
-  orc_beads_lubforce Newton's third law, line of centres, cut-off continuity
+  orc_beads_lubforce Newton's third law, line of centres, cut-off continuity;
+                     sphere-wall closed form and mirror symmetry (walls_y)
+  walls_y            placement clear of the walls; Poiseuille profile of the
+                     walled channel under a body force
   orc_beads_filling  uniform equilibrium fluid: refill = feq(rho, rigid-body u)
   orc_initpartvel    trilinear interpolation exact on a linear field
   orc_initpartpos    placement has no overlaps (with min gap); seeded repeatably
@@ -378,6 +381,70 @@ def test_body_force_accelerates_fluid_uniformly(oracle_lib):
         for comp, Fc in zip((o.ux, o.uy, o.uz), F):
             assert np.allclose(comp, n * Fc, rtol=1e-12, atol=1e-20)
         assert np.max(np.abs(o.rho)) < 1e-16
+
+
+def test_wall_lubrication_closed_form(oracle_lib):
+    """Channel walls: a sphere at distance d < rad + mingap_w from a wall is
+    pushed away with ((rad + mingap_w - d)/mingap_w)^2 / stf0_w * fpc (plus the
+    overlap term), mirror-symmetric at the two walls, nothing in x and z."""
+    prm = LBMParams(nx=32, ny=40, nz=32, visc=0.01, npart=2, rad=4.0, walls_y=True,
+                    mingap_w=3.0, stf0_w=0.03, stf1_w=0.004)
+    o = _orc(prm)
+    d = 5.5
+    o.yp[:] = [[8.0, d, 9.0], [24.0, prm.ny - d, 21.0]]
+    o.ypglb[:] = o.yp
+    o.beads_lubforce()
+    fpc = prm.volp * prm.rho0 * 9.8 / prm.gscale
+    t = (prm.rad + prm.mingap_w - d) / prm.mingap_w
+    want = t * t / prm.stf0_w * fpc
+    assert abs(o.flubp[0, 1] - want) <= 1e-15 * want
+    assert abs(o.flubp[1, 1] + want) <= 1e-15 * want
+    assert np.all(o.flubp[:, [0, 2]] == 0.0)
+    # overlapping the wall adds the stiff term
+    o.yp[0, 1] = o.ypglb[0, 1] = 3.5
+    o.beads_lubforce()
+    t = (prm.rad + prm.mingap_w - 3.5) / prm.mingap_w
+    want = (t * t / prm.stf0_w + 0.5 / prm.mingap_w / prm.stf1_w) * fpc
+    assert abs(o.flubp[0, 1] - want) <= 1e-15 * want
+
+
+def test_channel_placement_clear_of_walls(oracle_lib):
+    prm = LBMParams(nx=40, ny=40, nz=40, visc=0.01, npart=10, rad=3.0, walls_y=True, mingap_w=4.0)
+    o = _orc(prm)
+    o.S.istep00 = 0
+    o.initpartpos()
+    y = o.ypglb[:, 1]
+    assert np.all(y >= prm.rad + prm.mingap_w) and np.all(y <= prm.ny - prm.rad - prm.mingap_w)
+    for a in range(prm.npart):
+        for b in range(a + 1, prm.npart):
+            dv = o.ypglb[a] - o.ypglb[b]
+            for k in (0, 2):
+                dv[k] -= 40 * np.round(dv[k] / 40)
+            assert np.linalg.norm(dv) - 2 * prm.rad - prm.mingap >= 0.0
+
+
+def test_channel_poiseuille_profile(oracle_lib):
+    """Half-way bounce-back walls at y = 0 and y = H with a uniform x body
+    force reach the Poiseuille profile u(y) = F y (H - y) / (2 nu) at the node
+    centres y = j + 1/2, up to a uniform wall slip (the MRT rate of the
+    energy-flux moments differs from the viscous one, so the bounce-back wall
+    sits slightly off mid-link; the slip is small and the same at every node)."""
+    H, F = 16, 1e-6
+    prm = LBMParams(nx=4, ny=H, nz=4, visc=0.1, forcing=False, walls_y=True, body_force=(F, 0.0, 0.0))
+    o = _orc(prm)
+    o.initpop()
+    o.rhoupdat()
+    for n in range(1, 4001):
+        o.S.istep = n
+        o.call("timestep")
+    y = np.arange(H) + 0.5
+    want = F * y * (H - y) / (2 * prm.visc)
+    got = o.ux[0, :, 0]
+    slip = got - want
+    assert np.max(np.abs(slip - slip.mean())) < 1e-6 * want.max()    # the parabola
+    assert abs(slip.mean()) < 5e-3 * want.max()
+    assert np.allclose(o.ux, o.ux[0:1, :, 0:1], rtol=0, atol=1e-15)   # uniform in x and z
+    assert np.max(np.abs(o.uy)) < 1e-15 and np.max(np.abs(o.uz)) < 1e-15
 
 
 def test_lubrication_pair_properties(oracle_lib):
