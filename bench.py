@@ -29,21 +29,22 @@ BYTES_PER_CELL = {"fp32": 152, "fp32x": 152, "fp64": 304}   # 19 populations rea
 # HBM bytes per k_collide dispatch, measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on this workload (FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0, the
 # factors a plain 1 GiB device copy gives): profiles/hbm_traffic_config2_256cube_300spheres.md
-MEASURED_TRAFFIC = {("config2", "fp32"): 2.60276e9, ("config2", "fp64"): 5.15306e9}
+MEASURED_TRAFFIC = {("config2", "fp32", "ab"): 2.60276e9, ("config2", "fp64", "ab"): 5.15306e9}
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_gpu(case, precision, steps, warmup, field, case_name):
+def run_gpu(case, precision, steps, warmup, field, case_name, storage="ab"):
     from multiphase_flow_codes_amd.workloads import setup_lattice
     t0 = time.time()
-    lat, prm, field = setup_lattice(case, precision, field)
-    log(f"{precision}: set-up {time.time() - t0:.1f} s")
+    lat, prm, field = setup_lattice(case, precision, field, storage=storage)
+    tag = f"{case_name} {precision}{'' if storage == 'ab' else ' ' + storage.upper()}"
+    log(f"{tag}: set-up {time.time() - t0:.1f} s")
     lat.step(1, warmup)
     lat.sync()
-    log(f"{precision}: warm-up {warmup} steps done")
+    log(f"{tag}: warm-up {warmup} steps done")
     lat.set_timing(True)
     lat.sync()
     t1 = time.perf_counter()
@@ -58,16 +59,17 @@ def run_gpu(case, precision, steps, warmup, field, case_name):
     mlups = ncell * steps / (t2 - t1) / 1e6
     avg_kern = kern_ms / max(launches, 1)
     achieved = BYTES_PER_CELL[precision] * ncell / (avg_kern * 1e-3) / 1e9
-    log(f"{precision}: {ms_step:.3f} ms/step, {mlups:.0f} MLUPS, collide {avg_kern:.3f} ms "
+    log(f"{tag}: {ms_step:.3f} ms/step, {mlups:.0f} MLUPS, collide {avg_kern:.3f} ms "
         f"({achieved:.0f} GB/s), links {st['nlinks']}, refilled nodes {st['refill']}")
     lat.close()
     return {
         "mlups": mlups, "ms_per_step": ms_step, "kernel_ms": avg_kern, "launches": launches,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC.get((case_name, precision)),
+                     "traffic": MEASURED_TRAFFIC.get((case_name, precision, storage)),
                      "kernel": {"fp32": "k_collide<float,float>", "fp32x": "k_collide<float,double>",
-                                "fp64": "k_collide<double,double>"}[precision],
+                                "fp64": "k_collide<double,double>"}[precision]
+                               + ("" if storage == "ab" else " (in-place AA layouts)"),
                      "bytes_per_cell": BYTES_PER_CELL[precision]},
         "links": st["nlinks"],
     }, field
@@ -159,15 +161,35 @@ def run_serving_bench(ncases, cap, rate, window=500, tol=1e-4):
             "final_drag_rel_err_vs_fp64": {"max": float(np.max(rel)), "median": float(np.median(rel))}}
 
 
+def run_config3(steps, warmup, fp32_aa=False, fp64=True):
+    """The 1024^3 channel with 20,000 spheres: FP32 shifted populations with
+    two buffers (163 GB) beside FP64 with one buffer updated in place
+    (AA pattern, 163 GB); two FP64 buffers would need 326 GB."""
+    from multiphase_flow_codes_amd import workloads
+    case = workloads.CONFIG3
+    keep = ("mlups", "ms_per_step", "kernel_ms", "roofline", "links")
+    r32, _ = run_gpu(case, "fp32", steps, warmup, None, "config3")
+    out = {"workload": case.describe(), "cells": case.n ** 3, "particles": case.npart,
+           "steps": steps, "warmup": warmup,
+           "fp32": {k: r32[k] for k in keep} | {"storage": "two buffers (A/B), 163 GB"}}
+    if fp32_aa:
+        ra, _ = run_gpu(case, "fp32", steps, warmup, None, "config3", storage="aa")
+        out["fp32_aa"] = {k: ra[k] for k in keep} | {"storage": "one buffer in place (AA), 82 GB"}
+    if fp64:
+        r64, _ = run_gpu(case, "fp64", steps, warmup, None, "config3", storage="aa")
+        out["fp64"] = {k: r64[k] for k in keep} | {"storage": "one buffer in place (AA), 163 GB"}
+        out["fp32_over_fp64"] = round(r32["mlups"] / r64["mlups"], 3)
+    return out
+
+
 def run_cpu_baseline(case, field, nsteps):
-    from multiphase_flow_codes_amd.workloads import BoxCase  # noqa: F401
     from oracle import pyoracle
     import subprocess
     if not os.path.exists(pyoracle.LIB_PATH):
         subprocess.check_call(["bash", os.path.join(ROOT, "oracle", "build.sh")])
     prm = case.params()
     o = pyoracle.Oracle(prm, bb_mode=0)
-    o.ux[:], o.uy[:], o.uz[:] = field
+    o.ux[:], o.uy[:], o.uz[:] = case.host_field() if field is None else field
     o.initpop()
     o.rhoupdat()
     o.macrovar()
@@ -226,6 +248,13 @@ def main():
                     help="skip the FP32-storage / FP64-collision variant")
     ap.add_argument("--no-fp32", action="store_true",
                     help="skip the FP32 path (profiling runs of the FP64 kernels only)")
+    ap.add_argument("--config3-steps", type=int, default=100,
+                    help="steps of the 1024^3 channel leg (FP32 A/B and FP64 AA; 0: skip)")
+    ap.add_argument("--config3-only", action="store_true",
+                    help="run only the 1024^3 channel leg (profiling)")
+    ap.add_argument("--config3-fp32-aa", action="store_true",
+                    help="also run the 1024^3 channel in FP32 with in-place storage")
+    ap.add_argument("--config3-no-fp64", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized buffer 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -238,6 +267,10 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
     from multiphase_flow_codes_amd import workloads
+    if args.config3_only:
+        log(json.dumps(run_config3(args.config3_steps, min(args.warmup, 10), args.config3_fp32_aa,
+                                   not args.config3_no_fp64)))
+        return
     case = workloads.CONFIG2 if args.case == "config2" else workloads.CONFIG1
     r32 = r64 = field = None
     if not args.no_fp32:
@@ -256,6 +289,7 @@ def main():
                            ("fp32",) if args.no_fp32x else ("fp32", "fp32x"))
     ens = None if args.no_ensemble else run_ensemble_bench(args.ens_cases, args.ens_steps)
     srv = None if args.no_serving else run_serving_bench(args.serve_cases, args.serve_cap, args.serve_rate)
+    c3 = run_config3(args.config3_steps, min(args.warmup, 10)) if args.config3_steps > 0 else None
     cpu = None
     if not args.no_cpu:
         cpu = run_cpu_baseline(case, field, args.cpu_steps)
@@ -294,6 +328,8 @@ def main():
         out["serving"] = srv
         out["p50_case_latency_s"] = srv["engine_fp32"]["p50_s"]
         out["p99_case_latency_s"] = srv["engine_fp32"]["p99_s"]
+    if c3 is not None:
+        out["config3"] = c3
     if acc is not None:
         out["accuracy_fp32_vs_fp64"] = acc["fp32"]
         if "fp32x" in acc:

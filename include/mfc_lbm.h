@@ -45,6 +45,10 @@ extern "C" {
 #define MFC_PREC_FP32 1          /* populations stored and collided in FP32 */
 #define MFC_PREC_FP32_F64MATH 2  /* populations stored in FP32, collided in FP64 */
 
+#define MFC_STORE_AB 0  /* two population buffers (2 x 19 values per node) */
+#define MFC_STORE_AA 1  /* one buffer updated in place (AA pattern): 19 values per node,
+                           e.g. a 1024^3 FP64 box in 163 GB; not for ensembles */
+
 /* Run constants: what the reference holds in module var_inc after `para`
  * (BOX/para.f90:1-288, BOX/var_inc.f90:14-29). */
 typedef struct mfc_params {
@@ -54,6 +58,7 @@ typedef struct mfc_params {
   int32_t walls_y;             /* 1: half-way bounce-back walls on the y faces */
   int32_t iseedf;              /* forcing RNG state after para (negative: reseed), para.f90:281 */
   int32_t irelease;            /* release step (var_inc.f90:24) */
+  int32_t storage;             /* MFC_STORE_AB or MFC_STORE_AA */
   double s1, s2, s4, s9, s10, s13, s16;   /* MRT rates (para.f90:113-135) */
   double omegepsl, omegepslj, omegxx;     /* MRT equilibrium weights (para.f90:102-111) */
   double ww0, ww1, ww2;                   /* lattice weights (para.f90:167-169) */
@@ -105,6 +110,14 @@ int mfc_set_forcing_state(mfc_ctx* ctx, int32_t iseedf, int32_t iyf, const int32
                           const double* a1r, const double* a2r, const double* a3r);
 
 /* ---- fluid step (BOX/main.f90:172-188) ----------------------------------- */
+/* Synthetic initial velocity (no counterpart in the reference, which starts
+ * from rest, BOX/main.f90:73-75; it replaces the upload of a host-generated
+ * field before initpop): ux/uy/uz = scale * sum over the table of
+ * 2 Re(a_c(k) exp(2 pi i (kx x/nx + ky y/ny + kz z/nz))) at the node indices
+ * x, y, z = 0..n-1, rho = 0.  amp holds (re, im) of a_c for 0 <= kx <= 16 and
+ * -16 <= ky, kz <= 16 at double index
+ * 2 * (kx + 17 * ((ky + 16) + 33 * ((kz + 16) + 33 * c))) (+1 for im), c = 0..2. */
+int mfc_init_velocity(mfc_ctx* ctx, const double* amp, double scale);
 int mfc_initpop(mfc_ctx* ctx);          /* BOX/initial.f90:3-28 */
 int mfc_forcing(mfc_ctx* ctx);          /* BOX/gaussian.f90:182-372 */
 int mfc_collision_mrt(mfc_ctx* ctx);    /* BOX/collision.f90:8-210 */

@@ -7,6 +7,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -27,6 +28,7 @@ SOURCES = {
     "particles.hip": ["-ffp-contract=off"],
     "diag.hip": [],
     "ensemble.hip": [],
+    "initial.hip": [],
 }
 
 
@@ -42,14 +44,16 @@ def build_library(verbose: bool = False) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src, extra in SOURCES.items():
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
-        cmd = [hipcc, *COMMON, *extra, "-c", "-o", obj, os.path.join(CSRC, src)]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        _run(cmd)
+        cmds.append([hipcc, *COMMON, *extra, "-c", "-o", obj, os.path.join(CSRC, src)])
         objs.append(obj)
+        if verbose:
+            print(" ".join(cmds[-1]), flush=True)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "4"))))
+    with ThreadPoolExecutor(jobs) as pool:
+        list(pool.map(_run, cmds))
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs])
     os.replace(tmp, LIB)

@@ -293,6 +293,7 @@ int mfc_create(const mfc_params* p, mfc_ctx** out) {
   if (p->precision != MFC_PREC_FP64 && p->precision != MFC_PREC_FP32 &&
       p->precision != MFC_PREC_FP32_F64MATH)
     return MFC_ERR_ARG;
+  if (p->storage != MFC_STORE_AB && p->storage != MFC_STORE_AA) return MFC_ERR_ARG;
   mfc_ctx* ctx = new (std::nothrow) mfc_ctx();
   if (!ctx) return MFC_ERR_ARG;
   Ctx& c = ctx->c;
@@ -302,6 +303,8 @@ int mfc_create(const mfc_params* p, mfc_ctx** out) {
   c.g.nz = p->nz;
   c.g.walls_y = p->walls_y ? 1 : 0;
   c.g.ncell = ncell;
+  // in-place storage starts with the streamed field in each node's own slots
+  c.g.lay = p->storage == MFC_STORE_AA ? LAY_AA_ODD : LAY_AB;
   c.cc = CollideConsts{p->s1, p->s2, p->s4, p->s9, p->s10, p->s13, p->s16, p->omegepsl, p->omegepslj,
                        p->omegxx, p->ww0, p->ww1, p->ww2, p->body_force[0], p->body_force[1],
                        p->body_force[2]};
@@ -322,9 +325,11 @@ int mfc_create(const mfc_params* p, mfc_ctx** out) {
   c.own_stream = true;
   const size_t pb = (size_t)19 * ncell * c.tsize;
   CK(hipMalloc(&c.pop[0], pb));
-  CK(hipMalloc(&c.pop[1], pb));
   CK(hipMemsetAsync(c.pop[0], 0, pb, c.stream));
-  CK(hipMemsetAsync(c.pop[1], 0, pb, c.stream));
+  if (c.g.lay == LAY_AB) {
+    CK(hipMalloc(&c.pop[1], pb));
+    CK(hipMemsetAsync(c.pop[1], 0, pb, c.stream));
+  }
   const size_t gb = sizeof(double) * 18 * (size_t)p->ny * p->nz;
   for (int s = 0; s < kForceSlots; ++s) CK(hipMalloc(&c.ft.g[s], gb));
   c.ft.cur = 0;
@@ -547,6 +552,30 @@ int mfc_initpop(mfc_ctx* ctx) {
   return MFC_OK;
 }
 
+int mfc_init_velocity(mfc_ctx* ctx, const double* amp, double scale) {
+  NEEDCTX;
+  if (!amp) return MFC_ERR_ARG;
+  Ctx& c = ctx->c;
+  int rc = ensure_macro(ctx);
+  if (rc) return rc;
+  const size_t nk = (size_t)init_mode_k(), nkx = (size_t)init_mode_kx();
+  const size_t na = 3 * nkx * nk * nk;
+  const size_t n1 = nk * nk * (size_t)c.g.nx, n2 = nk * (size_t)c.g.nx * (size_t)c.g.ny;
+  double2 *d_amp = nullptr, *B1 = nullptr, *B2 = nullptr;
+  hipError_t e = hipMalloc(&d_amp, sizeof(double2) * na);
+  if (e == hipSuccess) e = hipMalloc(&B1, sizeof(double2) * n1);
+  if (e == hipSuccess) e = hipMalloc(&B2, sizeof(double2) * n2);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_amp, amp, sizeof(double2) * na, hipMemcpyHostToDevice, c.stream);
+  if (e == hipSuccess) e = launch_init_velocity(c, d_amp, scale, B1, B2);
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  if (d_amp) hipFree(d_amp);
+  if (B1) hipFree(B1);
+  if (B2) hipFree(B2);
+  if (e != hipSuccess) return fail(ctx, MFC_ERR_HIP, "mfc_init_velocity", e);
+  c.mom = MOM_ARRAYS;
+  return MFC_OK;
+}
+
 int mfc_forcing(mfc_ctx* ctx) {
   NEEDCTX;
   Ctx& c = ctx->c;
@@ -580,7 +609,10 @@ int mfc_collision_mrt(mfc_ctx* ctx) {
   } else {
     HIPCHK(launch_collide(c, arrays));
   }
-  c.cur ^= 1;
+  if (c.g.lay == LAY_AB)
+    c.cur ^= 1;
+  else
+    c.g.lay = c.g.lay == LAY_AA_ODD ? LAY_AA_EVEN : LAY_AA_ODD;
   c.phase = PHASE_PRE_STREAM;
   c.f_version++;
   return MFC_OK;

@@ -97,6 +97,7 @@ struct StageTab {
 int check_member(mfc_ens* e, Ctx& c) {
   if (c.p.xkf > 0.0) return efail(e, MFC_ERR_ARG, "ensemble members are driven by a body force only (xkf = 0)");
   if (c.phase != PHASE_POST_STREAM) return efail(e, MFC_ERR_STATE, "ensemble member not at the end of a step");
+  if (c.g.lay != LAY_AB) return efail(e, MFC_ERR_ARG, "ensemble members use two-buffer storage (MFC_STORE_AB)");
   if (!(c.mom == MOM_ARRAYS || (c.mom == MOM_VIRTUAL && c.mom_version == c.f_version)))
     return efail(e, MFC_ERR_STATE, "ensemble member needs macrovar before joining");
   if (c.pt.np > 0 && c.pt.map_np[c.pt.map_cur] == 0)

@@ -68,10 +68,18 @@ __device__ __forceinline__ void force_row(const double* __restrict__ t, R c1, R 
 // flow, used while they are authoritative, e.g. the pre-relaxation);
 // otherwise recomputed from the pulled populations plus half the forcing
 // captured at the last macrovar, which is exactly what those arrays hold.
-template <typename T, typename R, bool ARR, bool WALLS>
+//
+// LAY: layout of the input field.  LAY_AB reads buffer A and writes buffer B.
+// The in-place (AA) layouts use one buffer: every node reads its 19 streamed
+// values and writes post-collision value i where it read opp(i), so each
+// node touches exactly the locations it read and nodes never race.  From
+// LAY_AA_ODD that is all in the node's own slots; from LAY_AA_EVEN it is the
+// neighbours' (the result is in LAY_AA_ODD, and the other way round).
+template <typename T, typename R, bool ARR, bool WALLS, int LAY>
 __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y, int z) {
   const T* __restrict__ A = (const T*)a.A;
   T* __restrict__ B = (T*)a.B;
+  T* F = (T*)a.B;   // the one buffer of the in-place layouts (A == B)
   const uint32_t* __restrict__ mask = a.mask;
   const Grid& g = a.g;
   const CollideConsts& k = a.k;
@@ -91,14 +99,28 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
   const uint32_t xs[3] = {xpp, (uint32_t)x, xm};   // cx = -1, 0, +1
   const uint32_t n = (uint32_t)x + nx * row;
   const bool wall_lo = WALLS && y == 0, wall_hi = WALLS && y == g.ny - 1;
+  // location of the streamed value of direction i
+  auto src_of = [&](int i, int& pl, uint32_t& src) {
+    if (LAY == LAY_AA_ODD) {
+      pl = i;
+      src = n;
+      return;
+    }
+    const int ccx = cx(i), ccy = cy(i), ccz = cz(i);
+    const bool wall = (ccy > 0 && wall_lo) || (ccy < 0 && wall_hi);
+    src = wall ? n : xs[ccx + 1] + nx * (yy[ccy + 1] + ny * zz[ccz + 1]);
+    if (LAY == LAY_AB)
+      pl = wall ? opp(i) : i;
+    else
+      pl = wall ? i : opp(i);
+  };
   R f[NPOP];
 #pragma unroll
   for (int i = 0; i < NPOP; ++i) {
-    const int ccx = cx(i), ccy = cy(i), ccz = cz(i);
-    const bool wall = (ccy > 0 && wall_lo) || (ccy < 0 && wall_hi);
-    const uint32_t src = wall ? n : xs[ccx + 1] + nx * (yy[ccy + 1] + ny * zz[ccz + 1]);
-    const int pl = wall ? opp(i) : i;
-    f[i] = (R)A[(uint64_t)pl * g.ncell + src];
+    int pl;
+    uint32_t src;
+    src_of(i, pl, src);
+    f[i] = LAY == LAY_AB ? (R)A[(uint64_t)pl * g.ncell + src] : (R)F[(uint64_t)pl * g.ncell + src];
   }
   const bool solid = mask != nullptr && ((mask[n >> 5] >> (n & 31u)) & 1u);
   if (!solid) {
@@ -141,15 +163,25 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
     else
       mrt_collide<R, false>(f, rho, ux, uy, uz, Fx, Fy, Fz, k);
   }
+  if (LAY == LAY_AB) {
 #pragma unroll
-  for (int i = 0; i < NPOP; ++i) __builtin_nontemporal_store((T)f[i], &B[(uint64_t)i * g.ncell + n]);
+    for (int i = 0; i < NPOP; ++i) __builtin_nontemporal_store((T)f[i], &B[(uint64_t)i * g.ncell + n]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < NPOP; ++i) {
+      int pl;
+      uint32_t dst;
+      src_of(opp(i), pl, dst);
+      __builtin_nontemporal_store((T)f[i], &F[(uint64_t)pl * g.ncell + dst]);
+    }
+  }
 }
 
-template <typename T, typename R, bool ARR, bool WALLS>
+template <typename T, typename R, bool ARR, bool WALLS, int LAY>
 __global__ void __launch_bounds__(256) k_collide(CollideArgs a) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= a.g.nx) return;
-  collide_node<T, R, ARR, WALLS>(a, x, blockIdx.y, blockIdx.z);
+  collide_node<T, R, ARR, WALLS, LAY>(a, x, blockIdx.y, blockIdx.z);
 }
 
 // Several boxes in one grid (ensemble engine): block b belongs to box c with
@@ -176,9 +208,9 @@ __global__ void __launch_bounds__(BS)
   const uint32_t x = (uint32_t)(n % nx);
   const uint32_t r = (uint32_t)(n / nx);
   if (a.arr)
-    collide_node<T, R, true, WALLS>(a, (int)x, (int)(r % ny), (int)(r / ny));
+    collide_node<T, R, true, WALLS, LAY_AB>(a, (int)x, (int)(r % ny), (int)(r / ny));
   else
-    collide_node<T, R, false, WALLS>(a, (int)x, (int)(r % ny), (int)(r / ny));
+    collide_node<T, R, false, WALLS, LAY_AB>(a, (int)x, (int)(r % ny), (int)(r / ny));
 }
 
 // Forcing table of one step: per (y,z) row the nine partial sums
@@ -253,7 +285,7 @@ __global__ void __launch_bounds__(256)
   if (PULLED)
     load_pulled<T, double, WALLS>(A, g, x, y, z, n, f);
   else
-    load_direct<T, double>(A, g, n, f);
+    load_direct<T, double, WALLS>(A, g, x, y, z, n, f);
   double r, jx, jy, jz;
   density_momentum<double>(f, r, jx, jy, jz);
   double Fx = fa.bfx, Fy = fa.bfy, Fz = fa.bfz;
@@ -302,7 +334,7 @@ __global__ void __launch_bounds__(256)
     if (PULLED)
       load_pulled<T, double, WALLS>(A, g, x, y, z, n, f);
     else
-      load_direct<T, double>(A, g, n, f);
+      load_direct<T, double, WALLS>(A, g, x, y, z, n, f);
     double r = f[0];
 #pragma unroll
     for (int i = 1; i < NPOP; ++i) r = r + f[i];
@@ -356,7 +388,7 @@ __global__ void __launch_bounds__(256)
   if (pulled)
     load_pulled<T, double, WALLS>(A, g, x, y, z, n, f);
   else
-    load_direct<T, double>(A, g, n, f);
+    load_direct<T, double, WALLS>(A, g, x, y, z, n, f);
 #pragma unroll
   for (int i = 0; i < NPOP; ++i) dst[19ull * q + i] = f[i];
 }
@@ -429,7 +461,7 @@ static void row_launch(const Grid& g, dim3& grid, dim3& blk) {
 CollideArgs collide_args(Ctx& c) {
   CollideArgs a;
   a.A = c.pop[c.cur];
-  a.B = c.pop[c.cur ^ 1];
+  a.B = c.g.lay == LAY_AB ? c.pop[c.cur ^ 1] : c.pop[c.cur];
   a.mask = (c.pt.np > 0 && c.pt.map_np[c.pt.map_cur] > 0) ? c.pt.mask[c.pt.map_cur] : nullptr;
   a.g = c.g;
   a.k = c.cc;
@@ -439,21 +471,31 @@ CollideArgs collide_args(Ctx& c) {
   return a;
 }
 
-template <typename T, typename R>
-static void collide_dispatch(Ctx& c, bool arrays, const CollideArgs& a) {
+template <typename T, typename R, int LAY>
+static void collide_dispatch_lay(Ctx& c, bool arrays, const CollideArgs& a) {
   dim3 grid, blk;
   row_launch(c.g, grid, blk);
   if (arrays) {
     if (c.g.walls_y)
-      hipLaunchKernelGGL((k_collide<T, R, true, true>), grid, blk, 0, c.stream, a);
+      hipLaunchKernelGGL((k_collide<T, R, true, true, LAY>), grid, blk, 0, c.stream, a);
     else
-      hipLaunchKernelGGL((k_collide<T, R, true, false>), grid, blk, 0, c.stream, a);
+      hipLaunchKernelGGL((k_collide<T, R, true, false, LAY>), grid, blk, 0, c.stream, a);
   } else {
     if (c.g.walls_y)
-      hipLaunchKernelGGL((k_collide<T, R, false, true>), grid, blk, 0, c.stream, a);
+      hipLaunchKernelGGL((k_collide<T, R, false, true, LAY>), grid, blk, 0, c.stream, a);
     else
-      hipLaunchKernelGGL((k_collide<T, R, false, false>), grid, blk, 0, c.stream, a);
+      hipLaunchKernelGGL((k_collide<T, R, false, false, LAY>), grid, blk, 0, c.stream, a);
   }
+}
+
+template <typename T, typename R>
+static void collide_dispatch(Ctx& c, bool arrays, const CollideArgs& a) {
+  if (c.g.lay == LAY_AA_ODD)
+    collide_dispatch_lay<T, R, LAY_AA_ODD>(c, arrays, a);
+  else if (c.g.lay == LAY_AA_EVEN)
+    collide_dispatch_lay<T, R, LAY_AA_EVEN>(c, arrays, a);
+  else
+    collide_dispatch_lay<T, R, LAY_AB>(c, arrays, a);
 }
 
 hipError_t launch_collide(Ctx& c, bool arrays) {
@@ -504,7 +546,10 @@ static void rhoupdat_dispatch(Ctx& c, bool pulled, unsigned long long* mb) {
     else
       hipLaunchKernelGGL((k_rhoupdat<T, true, false>), grid, blk, 0, c.stream, A, c.g, c.rho, mb);
   } else {
-    hipLaunchKernelGGL((k_rhoupdat<T, false, false>), grid, blk, 0, c.stream, A, c.g, c.rho, mb);
+    if (c.g.walls_y)
+      hipLaunchKernelGGL((k_rhoupdat<T, false, true>), grid, blk, 0, c.stream, A, c.g, c.rho, mb);
+    else
+      hipLaunchKernelGGL((k_rhoupdat<T, false, false>), grid, blk, 0, c.stream, A, c.g, c.rho, mb);
   }
 }
 

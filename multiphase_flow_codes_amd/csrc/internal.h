@@ -165,6 +165,11 @@ hipError_t launch_initpartvel(Ctx& c);
 hipError_t launch_maps_to_host_layout(Ctx& c, int which, int32_t* ib, int32_t* is);
 hipError_t launch_links_dense(Ctx& c, int32_t* ilink, int32_t* mlink, double* alink);
 
+// ---- launchers (initial.hip) ----
+int init_mode_kx();   // kx values of the mode table (0..16)
+int init_mode_k();    // ky, kz values (-16..16)
+hipError_t launch_init_velocity(Ctx& c, const double2* amp, double scale, double2* B1, double2* B2);
+
 // ---- launchers (diag.hip) ----
 int diag_blocks();
 hipError_t launch_field_sums(Ctx& c, double* part, double* out);

@@ -1,8 +1,10 @@
 // lbm_access.h -- device-side access to the resident population buffer.
 //
-// The resident buffer always holds post-collision (pre-streaming) values;
-// "pulled" access reads f_i(x) = buf[i][x - c_i], i.e. the post-streaming
-// field, with half-way bounce-back at the y faces when walls are on
+// The resident buffer always holds post-collision (pre-streaming) values,
+// laid out as Grid::lay says (PopLayout).  "Pulled" access reads the streamed
+// value f_i(x), i.e. the post-collision value of direction i at x - c_i, with
+// half-way bounce-back at the y faces when walls are on: beyond a wall the
+// pull takes the node's own post-collision value of opp(i)
 // (CHAN/streaming.f90:175-187, moved from the x faces to the y faces).
 #pragma once
 #include "lbm_common.h"
@@ -18,25 +20,68 @@ __device__ __forceinline__ void node_coords(uint32_t n, const Grid& g, int& x, i
   z = (int)zz;
 }
 
-// Source of the pulled population i at node (x,y,z): plane index and node.
+__device__ __forceinline__ uint32_t node_wrapped(const Grid& g, int x, int y, int z) {
+  const int wx = x < 0 ? x + g.nx : (x >= g.nx ? x - g.nx : x);
+  const int wy = y < 0 ? y + g.ny : (y >= g.ny ? y - g.ny : y);
+  const int wz = z < 0 ? z + g.nz : (z >= g.nz ? z - g.nz : z);
+  return (uint32_t)wx + (uint32_t)g.nx * ((uint32_t)wy + (uint32_t)g.ny * (uint32_t)wz);
+}
+
+// Location (plane, slot) of the post-collision value of direction i at node
+// n = (x,y,z) in layout `lay`.
+template <bool WALLS>
+__device__ __forceinline__ void post_src(int lay, int i, int x, int y, int z, uint32_t n,
+                                         const Grid& g, int& plane, uint32_t& slot) {
+  if (lay == LAY_AB) {
+    plane = i;
+    slot = n;
+    return;
+  }
+  if (lay == LAY_AA_EVEN) {
+    plane = opp(i);
+    slot = n;
+    return;
+  }
+  const int ty = y + cy(i);
+  if (WALLS && (ty < 0 || ty >= g.ny)) {
+    plane = opp(i);
+    slot = n;
+    return;
+  }
+  plane = i;
+  slot = node_wrapped(g, x + cx(i), ty, z + cz(i));
+}
+
+// Location of the streamed value f_i at node (x,y,z): the post-collision value
+// of i at x - c_i, or beyond a wall that of opp(i) at the node itself.
+template <bool WALLS>
+__device__ __forceinline__ void pull_src_lay(int lay, int i, int x, int y, int z, uint32_t n,
+                                             const Grid& g, int& plane, uint32_t& slot) {
+  if (lay == LAY_AA_ODD) {   // x - c_i + c_i = x, and the wall image lands there too
+    plane = i;
+    slot = n;
+    return;
+  }
+  const int sy = y - cy(i);
+  const bool wall = WALLS && (sy < 0 || sy >= g.ny);
+  if (wall) {
+    plane = lay == LAY_AB ? opp(i) : i;
+    slot = n;
+    return;
+  }
+  plane = lay == LAY_AB ? i : opp(i);
+  slot = node_wrapped(g, x - cx(i), sy, z - cz(i));
+}
+
 template <bool WALLS>
 __device__ __forceinline__ void pull_src(int i, int x, int y, int z, uint32_t n, const Grid& g,
                                          int& plane, uint32_t& src) {
-  const int sx = x - cx(i), sy = y - cy(i), sz = z - cz(i);
-  const int wx = sx < 0 ? sx + g.nx : (sx >= g.nx ? sx - g.nx : sx);
-  const int wy = sy < 0 ? sy + g.ny : (sy >= g.ny ? sy - g.ny : sy);
-  const int wz = sz < 0 ? sz + g.nz : (sz >= g.nz ? sz - g.nz : sz);
-  plane = i;
-  src = (uint32_t)wx + (uint32_t)g.nx * ((uint32_t)wy + (uint32_t)g.ny * (uint32_t)wz);
-  if (WALLS && (sy < 0 || sy >= g.ny)) {  // half-way bounce-back (CHAN/streaming.f90:175-187)
-    plane = opp(i);
-    src = n;
-  }
+  pull_src_lay<WALLS>(g.lay, i, x, y, z, n, g, plane, src);
 }
 
 template <typename T, typename R, bool WALLS>
-__device__ __forceinline__ void load_pulled(const T* __restrict__ A, const Grid& g, int x, int y,
-                                            int z, uint32_t n, R (&f)[NPOP]) {
+__device__ __forceinline__ void load_pulled(const T* A, const Grid& g, int x, int y, int z,
+                                            uint32_t n, R (&f)[NPOP]) {
 #pragma unroll
   for (int i = 0; i < NPOP; ++i) {
     int pl;
@@ -46,27 +91,49 @@ __device__ __forceinline__ void load_pulled(const T* __restrict__ A, const Grid&
   }
 }
 
-template <typename T, typename R>
-__device__ __forceinline__ void load_direct(const T* __restrict__ A, const Grid& g, uint32_t n,
-                                            R (&f)[NPOP]) {
+// The post-collision values of node n (the reference's f between collision
+// and streaming).
+template <typename T, typename R, bool WALLS>
+__device__ __forceinline__ void load_direct(const T* A, const Grid& g, int x, int y, int z,
+                                            uint32_t n, R (&f)[NPOP]) {
 #pragma unroll
-  for (int i = 0; i < NPOP; ++i) f[i] = (R)A[(uint64_t)i * g.ncell + n];
+  for (int i = 0; i < NPOP; ++i) {
+    int pl;
+    uint32_t s;
+    post_src<WALLS>(g.lay, i, x, y, z, n, g, pl, s);
+    f[i] = (R)A[(uint64_t)pl * g.ncell + s];
+  }
 }
 
 // Write population i of node n into the resident buffer so that the current
-// phase reproduces it: directly (pre-streaming) or at its pull source
-// (post-streaming).
+// phase reproduces it: as its post-collision value (pre-streaming) or at its
+// pull source (post-streaming).
 template <typename T, bool WALLS>
-__device__ __forceinline__ void store_phase(T* __restrict__ A, const Grid& g, int i, int x, int y,
-                                            int z, uint32_t n, bool pulled, double v) {
-  if (!pulled) {
-    A[(uint64_t)i * g.ncell + n] = (T)v;
-    return;
-  }
+__device__ __forceinline__ void store_phase(T* A, const Grid& g, int i, int x, int y, int z,
+                                            uint32_t n, bool pulled, double v) {
   int pl;
   uint32_t s;
-  pull_src<WALLS>(i, x, y, z, n, g, pl, s);
+  if (pulled)
+    pull_src<WALLS>(i, x, y, z, n, g, pl, s);
+  else
+    post_src<WALLS>(g.lay, i, x, y, z, n, g, pl, s);
   A[(uint64_t)pl * g.ncell + s] = (T)v;
+}
+
+// Value of direction i at node n after collision, read by the bounce-back.
+template <typename T>
+__device__ __forceinline__ double post_value(const T* A, const Grid& g, int i, uint32_t n) {
+  if (g.lay == LAY_AB) return (double)A[(uint64_t)i * g.ncell + n];
+  int x, y, z;
+  uint32_t row;
+  node_coords(n, g, x, y, z, row);
+  int pl;
+  uint32_t s;
+  if (g.walls_y)
+    post_src<true>(g.lay, i, x, y, z, n, g, pl, s);
+  else
+    post_src<false>(g.lay, i, x, y, z, n, g, pl, s);
+  return (double)A[(uint64_t)pl * g.ncell + s];
 }
 
 }  // namespace mfc

@@ -53,10 +53,22 @@ struct CollideConsts {
   double bfx, bfy, bfz;                   // uniform body force added to the forcing field
 };
 
+// Population layouts of the resident buffer (where the post-collision value
+// of direction i at node n lives).
+//   LAY_AB       two buffers: the current one, plane i, slot n
+//   LAY_AA_EVEN  one buffer (in-place AA storage) after an even collide:
+//                plane opp(i), slot n
+//   LAY_AA_ODD   one buffer after an odd collide: plane i at the node
+//                n + c_i (plane opp(i), slot n where n + c_i is beyond a wall)
+// In LAY_AA_ODD the streamed value f_i(n) is plane i, slot n: the natural
+// layout an upload writes and the first collide reads in place.
+enum PopLayout { LAY_AB = 0, LAY_AA_EVEN = 1, LAY_AA_ODD = 2 };
+
 // Grid geometry handed to the kernels.
 struct Grid {
   int nx, ny, nz;
   int walls_y;        // 1: half-way bounce-back walls on the y faces
+  int lay;            // PopLayout of the resident buffer
   uint64_t ncell;     // nx*ny*nz
 };
 

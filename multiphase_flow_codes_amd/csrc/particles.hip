@@ -604,20 +604,20 @@ __device__ __forceinline__ void bounce_back_body(int b_, const T* __restrict__ A
     const bool sol1 = out1 || bit_at(mask, m1);
     const int qo = opp(q);
     const double wt = q <= 6 ? ww1 : ww2;
-    const double ff1 = (double)A[(uint64_t)q * g.ncell + n];
+    const double ff1 = post_value(A, g, q, n);
     double f9;
     if (alpha <= 0.5) {
       bool sol2 = true;
       double ff2a = 0.0, ff3a = 0.0;
       if (!sol1) {
         sol2 = out2 || bit_at(mask, m2);
-        ff2a = (double)A[(uint64_t)q * g.ncell + m1];
-        if (!sol2) ff3a = (double)A[(uint64_t)q * g.ncell + m2];
+        ff2a = post_value(A, g, q, m1);
+        if (!sol2) ff3a = post_value(A, g, q, m2);
       }
       f9 = bounce_back_value(alpha, ff1, ff2a, ff3a, sol1, sol2, 0.0, 0.0, wt, uwpro);
     } else {
-      const double ffo = (double)A[(uint64_t)qo * g.ncell + n];
-      const double ffo1 = sol1 ? 0.0 : (double)A[(uint64_t)qo * g.ncell + m1];
+      const double ffo = post_value(A, g, qo, n);
+      const double ffo1 = sol1 ? 0.0 : post_value(A, g, qo, m1);
       f9 = bounce_back_value(alpha, ff1, 0.0, 0.0, sol1, true, ffo, ffo1, wt, uwpro);
     }
     lk_out[l] = f9;
@@ -687,8 +687,16 @@ __device__ __forceinline__ void bb_scatter_body(int b_, T* __restrict__ A, Grid 
     node_coords(n, g, x, y, z, row);
     const int yt = y + cy(q);
     if (g.walls_y && (yt < 0 || yt >= g.ny)) continue;
-    const uint32_t t = (uint32_t)wrap1(x + cx(q), g.nx) + (uint32_t)g.nx * ((uint32_t)wrap1(yt, g.ny) + (uint32_t)g.ny * (uint32_t)wrap1(z + cz(q), g.nz));
-    A[(uint64_t)opp(q) * g.ncell + t] = (T)lk_out[l];
+    const int xt = wrap1(x + cx(q), g.nx), zt = wrap1(z + cz(q), g.nz);
+    const uint32_t t = (uint32_t)xt + (uint32_t)g.nx * ((uint32_t)wrap1(yt, g.ny) + (uint32_t)g.ny * (uint32_t)zt);
+    // the post-collision value of opp(q) at the solid node t
+    int pl;
+    uint32_t s;
+    if (g.walls_y)
+      post_src<true>(g.lay, opp(q), xt, wrap1(yt, g.ny), zt, t, g, pl, s);
+    else
+      post_src<false>(g.lay, opp(q), xt, wrap1(yt, g.ny), zt, t, g, pl, s);
+    A[(uint64_t)pl * g.ncell + s] = (T)lk_out[l];
   }
 }
 

@@ -85,6 +85,7 @@
       p%walls_y = 0
       p%iseedf = iseedf
       p%irelease = irelease
+      p%storage = MFC_STORE_AB
       p%s1 = s1
       p%s2 = s2
       p%s4 = s4

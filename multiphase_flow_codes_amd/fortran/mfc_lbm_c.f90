@@ -9,6 +9,7 @@
       integer(c_int), parameter :: MFC_OK = 0, MFC_ERR_OVERLAP = 4,     &
                                    MFC_ERR_CAPACITY = 5
       integer(c_int), parameter :: MFC_PREC_FP64 = 0, MFC_PREC_FP32 = 1
+      integer(c_int), parameter :: MFC_STORE_AB = 0, MFC_STORE_AA = 1
       integer(c_int), parameter :: MFC_PF_YPGLB = 0, MFC_PF_YPGLBP = 1, &
         MFC_PF_WP = 2, MFC_PF_WPP = 3, MFC_PF_OMGP = 4, MFC_PF_OMGPP = 5,&
         MFC_PF_FHIP = 6, MFC_PF_FLUBP = 7, MFC_PF_TORQP = 8,            &
@@ -17,7 +18,7 @@
 
       type, bind(C) :: mfc_params
         integer(c_int32_t) :: nx, ny, nz, precision, npart_max, walls_y
-        integer(c_int32_t) :: iseedf, irelease
+        integer(c_int32_t) :: iseedf, irelease, storage
         real(c_double) :: s1, s2, s4, s9, s10, s13, s16
         real(c_double) :: omegepsl, omegepslj, omegxx
         real(c_double) :: ww0, ww1, ww2, rho0

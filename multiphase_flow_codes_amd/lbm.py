@@ -23,6 +23,7 @@ MFC_OK = 0
 ERRORS = {1: "MFC_ERR_ARG", 2: "MFC_ERR_HIP", 3: "MFC_ERR_STATE", 4: "MFC_ERR_OVERLAP",
           5: "MFC_ERR_CAPACITY", 6: "MFC_ERR_NONFINITE", 7: "MFC_ERR_NODEVICE"}
 PREC = {"fp64": 0, "fp32": 1, "fp32x": 2}   # fp32x: FP32 storage, FP64 collision
+STORAGE = {"ab": 0, "aa": 1}                # two buffers, or one updated in place (AA pattern)
 PFIELDS = {"ypglb": 0, "ypglbp": 1, "wp": 2, "wpp": 3, "omgp": 4, "omgpp": 5, "fHIp": 6,
            "flubp": 7, "torqp": 8, "forcepp": 9, "torqpp": 10, "dwdt": 11, "domgdt": 12,
            "thetap": 13}
@@ -40,7 +41,7 @@ class MfcParams(C.Structure):
     _fields_ = [
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
         ("precision", C.c_int32), ("npart_max", C.c_int32), ("walls_y", C.c_int32),
-        ("iseedf", C.c_int32), ("irelease", C.c_int32),
+        ("iseedf", C.c_int32), ("irelease", C.c_int32), ("storage", C.c_int32),
         ("s1", C.c_double), ("s2", C.c_double), ("s4", C.c_double), ("s9", C.c_double),
         ("s10", C.c_double), ("s13", C.c_double), ("s16", C.c_double),
         ("omegepsl", C.c_double), ("omegepslj", C.c_double), ("omegxx", C.c_double),
@@ -76,6 +77,7 @@ SIGNATURES = {
     "mfc_get_forcing_state": [C.c_void_p, _I, _I, _I, _D, _D, _D, _D, _D, _D],
     "mfc_set_forcing_state": [C.c_void_p, C.c_int32, C.c_int32, _I, _D, _D, _D, _D, _D, _D],
     "mfc_initpop": [C.c_void_p],
+    "mfc_init_velocity": [C.c_void_p, _D, C.c_double],
     "mfc_forcing": [C.c_void_p],
     "mfc_collision_mrt": [C.c_void_p],
     "mfc_streaming": [C.c_void_p],
@@ -134,8 +136,10 @@ def lib():
     return _lib
 
 
-def make_params(prm: LBMParams, precision: str = "fp64", npart_max: int | None = None) -> MfcParams:
+def make_params(prm: LBMParams, precision: str = "fp64", npart_max: int | None = None,
+                storage: str = "ab") -> MfcParams:
     p = MfcParams()
+    p.storage = STORAGE[storage]
     p.nx, p.ny, p.nz = prm.nx, prm.ny, prm.nz
     p.precision = PREC[precision]
     p.npart_max = prm.npart if npart_max is None else npart_max
@@ -165,11 +169,12 @@ def _ip(a):
 class Lattice:
     """One box on the GPU (a context of libmfc_lbm)."""
 
-    def __init__(self, prm: LBMParams, precision: str = "fp64"):
+    def __init__(self, prm: LBMParams, precision: str = "fp64", storage: str = "ab"):
         self.prm = prm
         self.precision = precision
+        self.storage = storage
         self.shape = (prm.nz, prm.ny, prm.nx)
-        self._p = make_params(prm, precision)
+        self._p = make_params(prm, precision, storage=storage)
         h = C.c_void_p()
         rc = lib().mfc_create(C.byref(self._p), C.byref(h))
         if rc != MFC_OK:
@@ -249,6 +254,13 @@ class Lattice:
 
     # -- the reference subroutines ---------------------------------------------
     def initpop(self): self._chk(lib().mfc_initpop(self.h))
+
+    def init_velocity(self, amp, scale: float):
+        """Band-limited velocity field synthesised on the device from a dense
+        mode table (see initial.dense_modes), rho = 0; then initpop."""
+        a = np.ascontiguousarray(amp, dtype=np.complex128)
+        assert a.shape == (3, 33, 33, 17), a.shape
+        self._chk(lib().mfc_init_velocity(self.h, a.ctypes.data_as(_D), float(scale)))
     def forcing(self): self._chk(lib().mfc_forcing(self.h))
     def collision_mrt(self): self._chk(lib().mfc_collision_mrt(self.h))
     def streaming(self): self._chk(lib().mfc_streaming(self.h))

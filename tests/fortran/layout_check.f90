@@ -17,6 +17,7 @@
       call show('walls_y', c_loc(p%walls_y))
       call show('iseedf', c_loc(p%iseedf))
       call show('irelease', c_loc(p%irelease))
+      call show('storage', c_loc(p%storage))
       call show('s1', c_loc(p%s1))
       call show('s2', c_loc(p%s2))
       call show('s4', c_loc(p%s4))

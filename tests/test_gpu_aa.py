@@ -1,0 +1,156 @@
+"""In-place (AA-pattern) population storage against two-buffer storage and
+the FP64 oracle.
+
+One buffer alternates two layouts (after an even collide each node keeps its
+post-collision values in its own slots, swapped to the opposite direction;
+after an odd collide they sit in the neighbours' slots), so every consumer of
+the field -- bounce-back gathers and scatters, refilling, macrovar, the host
+transfers -- has to find each value in either layout.  The arithmetic is the
+same as with two buffers, so the FP64 state must agree bitwise with the
+two-buffer path and to 1e-12 with the oracle, after odd and even step counts,
+with spheres, forcing and channel walls.
+"""
+import numpy as np
+import pytest
+
+from helpers import place_spheres, small_params, smooth_velocity
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _released(prm, storage, precision, pos, seed):
+    from multiphase_flow_codes_amd.lbm import Lattice
+    lat = Lattice(prm, precision, storage=storage)
+    ux, uy, uz = smooth_velocity(prm, np.random.default_rng(seed), amp=0.02)
+    lat.upload_macro(None, ux, uy, uz)
+    lat.initpop()
+    lat.rhoupdat()
+    lat.macrovar()
+    if prm.npart:
+        lat.prt_init(prm.npart)
+        lat.prt_set("ypglb", pos)
+        lat.initpartvel()
+        lat.initpartomg()
+        lat.beads_links()
+    return lat
+
+
+def _fine_step(lat, it, parts):
+    lat.forcing()
+    lat.collision_mrt()
+    if parts:
+        lat.beads_collision()
+    pre = lat.download_f()          # the post-collision field (direct reads)
+    lat.streaming()
+    if parts:
+        lat.beads_lubforce()
+        lat.beads_move(it)
+        lat.beads_redistribute(it)
+        lat.beads_links()
+        lat.beads_filling()
+    lat.macrovar()
+    return pre
+
+
+CASES = {
+    "periodic": dict(n=(32, 32, 32), npart=3, rad=4.0, rhopart=1.5),
+    "channel": dict(n=(32, 28, 32), npart=3, rad=4.0, rhopart=2.0, walls=True, forcing=False,
+                    strong_forcing=False, body_force=(2e-5, 0.0, 0.0), mingap_w=3.0),
+}
+
+
+def _positions(name, prm, rng):
+    if name == "channel":
+        return np.array([[6.3, 5.2, 7.7], [20.1, 14.4, 18.9], [11.8, 22.6, 27.3]])
+    return place_spheres(prm, rng, prm.npart, gap=2.5)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_aa_fp64_bitwise_equal_to_two_buffers(name):
+    prm = small_params(**CASES[name])
+    pos = _positions(name, prm, np.random.default_rng(3))
+    ab = _released(prm, "ab", "fp64", pos, 7)
+    aa = _released(prm, "aa", "fp64", pos, 7)
+    assert np.array_equal(ab.download_f(), aa.download_f())
+    # routine by routine: both layouts of the post-collision field are read back
+    for it in range(1, 4):
+        pa = _fine_step(ab, it, True)
+        pb = _fine_step(aa, it, True)
+        assert np.array_equal(pa, pb), f"post-collision field, step {it}"
+        assert np.array_equal(ab.download_f(), aa.download_f()), f"streamed field, step {it}"
+    # the fused step, odd and even counts
+    for it0, k in ((4, 5), (9, 4)):
+        ab.step(it0, k)
+        aa.step(it0, k)
+        ab.sync()
+        aa.sync()
+        assert np.array_equal(ab.download_f(), aa.download_f()), (it0, k)
+        for field in ("ypglb", "wp", "omgp", "fHIp", "torqp", "flubp"):
+            assert np.array_equal(ab.prt_get(field), aa.prt_get(field)), field
+        for which in (0, 1):
+            ia, sa = ab.get_maps(which)
+            ib, sb = aa.get_maps(which)
+            assert np.array_equal(ia, ib) and np.array_equal(sa, sb)
+        for a, b in zip(ab.download_macro(), aa.download_macro()):
+            assert np.array_equal(a, b)
+    assert ab.status()["refill"] == aa.status()["refill"]
+
+
+def test_aa_fp64_matches_oracle_with_walls(oracle_lib):
+    from oracle.pyoracle import Oracle
+    prm = small_params(**CASES["channel"])
+    pos = _positions("channel", prm, None)
+    lat = _released(prm, "aa", "fp64", pos, 5)
+    orc = Oracle(prm, bb_mode=1)
+    ux, uy, uz = smooth_velocity(prm, np.random.default_rng(5), amp=0.02)
+    orc.ux[:], orc.uy[:], orc.uz[:] = ux, uy, uz
+    orc.initpop()
+    orc.rhoupdat()
+    orc.macrovar()
+    orc.ypglb[:] = pos
+    orc.yp[:] = pos
+    orc.S.istep00 = 1
+    orc.call("initpartvel")
+    orc.call("initpartomg")
+    orc.beads_links()
+    for it in range(1, 8):
+        orc.S.istep = it
+        orc.call("timestep")
+    lat.step(1, 7)
+    lat.sync()
+    assert rel(lat.download_f(), orc.f) < TOL64
+    for name in ("ypglb", "wp", "omgp", "fHIp", "torqp", "flubp"):
+        assert rel(lat.prt_get(name), getattr(orc, name)) < TOL64, name
+
+
+def test_aa_fp32_close_to_two_buffers():
+    prm = small_params(**CASES["periodic"])
+    pos = _positions("periodic", prm, np.random.default_rng(3))
+    ab = _released(prm, "ab", "fp32", pos, 9)
+    aa = _released(prm, "aa", "fp32", pos, 9)
+    ab.step(1, 9)
+    aa.step(1, 9)
+    ab.sync()
+    aa.sync()
+    fa, fb = ab.download_f(), aa.download_f()
+    assert rel(fb, fa) < 1e-6
+    assert rel(aa.prt_get("fHIp"), ab.prt_get("fHIp")) < 1e-5
+
+
+def test_aa_refused_by_ensemble():
+    from multiphase_flow_codes_amd.lbm import Ensemble, MfcError
+    prm = small_params(n=(32, 32, 32), forcing=False, strong_forcing=False)
+    lat = _released(prm, "aa", "fp32", None, 1)
+    ens = Ensemble()
+    try:
+        with pytest.raises(MfcError):
+            ens.add(lat, 1)
+    finally:
+        ens.close()
+        lat.close()
