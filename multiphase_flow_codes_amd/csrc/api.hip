@@ -260,6 +260,8 @@ int check_faults(mfc_ctx* ctx) {
     return fail(ctx, MFC_ERR_OVERLAP, buf);
   }
   if (s[ST_LINK_OVERFLOW] > 0) return fail(ctx, MFC_ERR_CAPACITY, "fault: boundary-link pool exceeded");
+  if (s[ST_CELL_OVERFLOW] > 0)
+    return fail(ctx, MFC_ERR_CAPACITY, "fault: too many particles in one pair-search cell (overlapping particles)");
   if (s[ST_NONFINITE] > 0 || s[ST_BOX_FAULT] > 0)
     return fail(ctx, MFC_ERR_NONFINITE, "fault: particle position not finite or outside the box");
   return MFC_OK;
@@ -372,14 +374,14 @@ int mfc_destroy(mfc_ctx* ctx) {
   F(c.ft.phx);
   Particles& P = c.pt;
   double* pd[] = {P.yp, P.ypglb, P.ypglbp, P.wp, P.wpp, P.omgp, P.omgpp, P.fHIp, P.flubp, P.torqp,
-                  P.forcep, P.forcepp, P.torqpp, P.dwdt, P.domgdt, P.thetap, P.lk_alpha, P.lk_out,
-                  P.seg_ft};
+                  P.forcep, P.forcepp, P.torqpp, P.dwdt, P.domgdt, P.thetap, P.lk_alpha, P.seg_ft};
   for (double* a : pd) F(a);
+  F(P.lk_out); F(P.ct_list); F(P.ct_count);
   F(P.isn[0]); F(P.isn[1]); F(P.mask[0]); F(P.mask[1]);
   F(P.lk_node); F(P.lk_dir); F(P.lk_cnt); F(P.lk_off);
   F(P.sol_list[0]); F(P.sol_list[1]); F(P.sol_cnt[0]); F(P.sol_cnt[1]);
   F(P.sh_node); F(P.sh_hits); F(P.sh_cnt);
-  F(P.fill_list); F(P.fill_count);
+  F(P.fill_list); F(P.fill_count); F(P.cell_cnt); F(P.cell_list);
   F(c.d_status); F(c.d_red); F(c.staging);
   F(c.diag_part); F(c.diag_out); F(c.diag_Z);
   for (int a = 0; a < 3; ++a) F(c.diag_tw[a]);
@@ -659,7 +661,7 @@ int mfc_prt_init(mfc_ctx* ctx, int32_t npart) {
   NEEDCTX;
   Ctx& c = ctx->c;
   Particles& P = c.pt;
-  if (npart < 0) return MFC_ERR_ARG;
+  if (npart < 0 || npart >= (1 << 24)) return MFC_ERR_ARG;   // contact-list entries hold 24-bit ids
   if (npart > 0) {
     const int need = 2 * (int)ceil(c.p.rad + 2.0) + 6;
     if (c.g.nx < need || c.g.nz < need || (!c.g.walls_y && c.g.ny < need))
@@ -718,10 +720,16 @@ int mfc_prt_init(mfc_ctx* ctx, int32_t npart) {
     if (P.lk_dir) HIPCHK(hipFree(P.lk_dir));
     if (P.lk_alpha) HIPCHK(hipFree(P.lk_alpha));
     if (P.lk_out) HIPCHK(hipFree(P.lk_out));
+    if (P.ct_list) HIPCHK(hipFree(P.ct_list));
+    if (!P.ct_count) {
+      HIPCHK(hipMalloc(&P.ct_count, sizeof(int32_t)));
+      HIPCHK(hipMemsetAsync(P.ct_count, 0, sizeof(int32_t), c.stream));
+    }
     HIPCHK(hipMalloc(&P.lk_node, sizeof(uint32_t) * cap));
     HIPCHK(hipMalloc(&P.lk_dir, sizeof(uint8_t) * cap));
     HIPCHK(hipMalloc(&P.lk_alpha, sizeof(double) * cap));
-    HIPCHK(hipMalloc(&P.lk_out, sizeof(double) * cap));
+    HIPCHK(hipMalloc(&P.lk_out, sizeof(double2) * cap));
+    HIPCHK(hipMalloc(&P.ct_list, sizeof(uint64_t) * cap));
     P.lk_cap = cap;
     P.np_max = npart;
   }
@@ -742,6 +750,28 @@ int mfc_prt_init(mfc_ctx* ctx, int32_t npart) {
   }
   P.map_cur = P.map_old = 0;
   P.np = npart;
+  // pair-search cells for many particles: cells at least as wide as the
+  // lubrication range 2r + mingap (which covers the overlap range 2r), three
+  // or more along every axis (distinct neighbours, bounded cell width)
+  if (P.cell_cnt) HIPCHK(hipFree(P.cell_cnt));
+  if (P.cell_list) HIPCHK(hipFree(P.cell_list));
+  P.cell_cnt = nullptr;
+  P.cell_list = nullptr;
+  P.cells_ok = false;
+  if (npart > kCellMinParticles) {
+    const double h = 2.0 * c.p.rad + std::max(c.p.mingap, 0.0) + 1e-3;
+    const int ns[3] = {c.g.nx, c.g.ny, c.g.nz};
+    bool ok = true;
+    for (int a = 0; a < 3; ++a) {
+      P.cell_n[a] = std::max(1, (int)floor((double)ns[a] / h));
+      if (P.cell_n[a] < 3) ok = false;
+    }
+    if (ok) {
+      const size_t ncell = (size_t)P.cell_n[0] * P.cell_n[1] * P.cell_n[2];
+      HIPCHK(hipMalloc(&P.cell_cnt, sizeof(int32_t) * ncell));
+      HIPCHK(hipMalloc(&P.cell_list, sizeof(int32_t) * ncell * kCellCap));
+    }
+  }
   if (npart > 0) {
     double* pd[] = {P.yp, P.ypglb, P.ypglbp, P.wp, P.wpp, P.omgp, P.omgpp, P.fHIp, P.flubp, P.torqp,
                     P.forcep, P.forcepp, P.torqpp, P.dwdt, P.domgdt, P.thetap};
@@ -782,7 +812,10 @@ int mfc_prt_set(mfc_ctx* ctx, int32_t field, const double* a) {
   if (!d || !a || P.np <= 0) return MFC_ERR_ARG;
   const size_t b = sizeof(double) * 3 * P.np;
   HIPCHK(hipMemcpyAsync(d, a, b, hipMemcpyHostToDevice, c.stream));
-  if (field == MFC_PF_YPGLB) HIPCHK(hipMemcpyAsync(P.yp, a, b, hipMemcpyHostToDevice, c.stream));
+  if (field == MFC_PF_YPGLB) {
+    HIPCHK(hipMemcpyAsync(P.yp, a, b, hipMemcpyHostToDevice, c.stream));
+    P.cells_ok = false;
+  }
   HIPCHK(hipStreamSynchronize(c.stream));
   return MFC_OK;
 }
@@ -845,6 +878,7 @@ int mfc_initpartpos(mfc_ctx* ctx, int32_t* iseedp, int32_t mingap_check) {
     if (round > 100000) return fail(ctx, MFC_ERR_CAPACITY, "initpartpos: no non-overlapping placement found");
   }
   *iseedp = idum;
+  P.cells_ok = false;
   HIPCHK(hipMemcpyAsync(P.ypglb, y.data(), sizeof(double) * y.size(), hipMemcpyHostToDevice, c.stream));
   HIPCHK(hipMemcpyAsync(P.yp, y.data(), sizeof(double) * y.size(), hipMemcpyHostToDevice, c.stream));
   HIPCHK(hipStreamSynchronize(c.stream));
@@ -983,7 +1017,7 @@ int mfc_step(mfc_ctx* ctx, int32_t istep, int32_t nsteps) {
     if (on) {
       // move with the periodic wrap of beads_redistribute fused in; links
       // with its overlap check fused in (reported by mfc_sync)
-      HIPCHK(launch_move(c, it == c.p.irelease ? 1 : 0, 1));
+      HIPCHK(launch_move(c, it == c.p.irelease ? 1 : 0, 1, true));
       P.map_old = P.map_cur;
       // links with the refill search in the expansion grid, then the refill
       if ((rc = links_now(ctx, 1, it, true))) return rc;

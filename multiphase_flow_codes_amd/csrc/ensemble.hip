@@ -33,6 +33,7 @@ namespace {
 constexpr int kStages = 8;          // collide + the seven particle stages
 constexpr int kRing = 3;            // pinned table buffers in flight
 constexpr int kFillBlocksPerBox = 32;
+constexpr int kContactBlocksPerBox = 1;   // contact pre-pass: contacts are rare
 
 struct Member {
   mfc_ctx* box;
@@ -110,7 +111,7 @@ int check_member(mfc_ens* e, Ctx& c) {
 int ens_one_step(mfc_ens* e) {
   StageTab coll, st[7];
   coll.arg_bytes = sizeof(CollideArgs);
-  const Stage order[7] = {SG_BOUNCE, SG_SCATTER, SG_MOVE, SG_SCAN, SG_COUNT, SG_EXPAND, SG_FILL};
+  const Stage order[7] = {SG_CONTACT, SG_BOUNCE, SG_MOVE, SG_SCAN, SG_COUNT, SG_EXPAND, SG_FILL};
   for (int k = 0; k < 7; ++k) st[k].arg_bytes = stage_arg_bytes(order[k]);
   for (Member& mb : e->m) {
     Ctx& c = mb.box->c;
@@ -132,16 +133,15 @@ int ens_one_step(mfc_ens* e) {
     if (on) {
       int nb = 0;
       const BounceArgs ba = bounce_args(c, true, &nb);
-      st[0].add(&ba, nb);
-      const ScatterArgs sa = scatter_args(c, true, &nb);
-      st[1].add(&sa, nb);
+      st[0].add(&ba, kContactBlocksPerBox);
+      st[1].add(&ba, nb);
       c.f_version++;
     }
     c.phase = PHASE_POST_STREAM;   // streaming
     c.f_version++;
     if (on) {
       int nb = 0;
-      const MoveKArgs ma = move_args(c, it == c.p.irelease ? 1 : 0, 1, &nb);
+      const MoveKArgs ma = move_args(c, it == c.p.irelease ? 1 : 0, 1, true, &nb);
       st[2].add(&ma, nb);
       P.map_old = P.map_cur;
       const int target = P.map_cur ^ 1;
@@ -282,6 +282,7 @@ int mfc_ens_remove(mfc_ens* e, mfc_ctx* box) {
     EHIP(hipStreamSynchronize(e->stream));
     box->c.stream = e->m[i].own_stream;
     box->c.ens = nullptr;
+    box->c.pt.cells_ok = false;   // the members' particles moved
     e->m.erase(e->m.begin() + (long)i);
     if (e->m.empty()) e->prec = e->walls = -1;
     return MFC_OK;

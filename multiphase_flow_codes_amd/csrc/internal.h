@@ -28,10 +28,16 @@ enum StatusWord {
   ST_NLINKS = 7,         // links built by the last beads_links
   ST_NSOLID = 8,         // solid nodes marked by the last beads_links
   ST_BOX_FAULT = 9,      // particle boxes that did not fit the lattice
+  ST_CELL_OVERFLOW = 10, // pair-search cells or neighbour lists over capacity
   ST_COUNT = 16
 };
 
 constexpr int kForceSlots = 4;
+// particle pair searches use a cell grid above this many particles; a cell
+// (between 1 and 4/3 times 2r + mingap wide) holds at most kCellCap centres
+// (non-overlapping spheres never come close to it; more is reported)
+constexpr int kCellMinParticles = 512;
+constexpr int kCellCap = 16;
 // The live forcing modes of one FORCING call, grouped by kx (n[0] modes with
 // kx = 0 first, then kx = 1, then kx = 2), each in (kz, ky) order, with
 // (re, im) of the three components.  Handed to the table kernel by value.
@@ -85,10 +91,18 @@ struct Particles {
   uint32_t* lk_node = nullptr;
   uint8_t* lk_dir = nullptr;
   double* lk_alpha = nullptr;
-  double* lk_out = nullptr;
+  double2* lk_out = nullptr;   // (f9, ff1) of contact links, taken before the scatter
+  uint64_t* ct_list = nullptr; // contact links of the last build: (particle << 40) | link
+  int32_t* ct_count = nullptr;
   int32_t* lk_cnt = nullptr;   // links per segment
   int64_t* lk_off = nullptr;   // link offset per segment (np * nseg + 1)
   double* seg_ft = nullptr;    // force and torque per segment (6 each)
+  // cell grid of the pair searches (on for many particles): per-cell counts
+  // and particle lists, valid for the positions at the last build
+  int cell_n[3] = {0, 0, 0};
+  int32_t* cell_cnt = nullptr;
+  int32_t* cell_list = nullptr;
+  bool cells_ok = false;
   // nodes to refill, found by beads_filling's first pass
   uint64_t* fill_list = nullptr;
   int32_t* fill_count = nullptr;
@@ -154,14 +168,17 @@ int segment_capacity(double rad);
 // runs in the same grid as the link expansion
 hipError_t launch_links(Ctx& c, int target, int overlap_check, int istep, bool with_find = false);
 // with `fused` set (mfc_step) the bounce-back grid also computes the
-// lubrication forces and the scatter grid clears the next map target
+// lubrication forces and clears the next map target, and fHIp / torqp are
+// left for the move kernel to sum from the segments
 hipError_t launch_bounce_back(Ctx& c, bool fused = false);
 hipError_t launch_lubforce(Ctx& c);
 hipError_t launch_map_clear(Ctx& c, int buf);
-hipError_t launch_move(Ctx& c, int first, int wrap);
+hipError_t launch_move(Ctx& c, int first, int wrap, bool from_seg = false);
 hipError_t launch_redistribute(Ctx& c, int istep);
 hipError_t launch_filling(Ctx& c, bool find = true);
 hipError_t launch_initpartvel(Ctx& c);
+// (re)build the pair-search cells if they are on and stale
+hipError_t ensure_cells(Ctx& c);
 hipError_t launch_maps_to_host_layout(Ctx& c, int which, int32_t* ib, int32_t* is);
 hipError_t launch_links_dense(Ctx& c, int32_t* ilink, int32_t* mlink, double* alink);
 

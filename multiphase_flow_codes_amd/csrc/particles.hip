@@ -64,6 +64,101 @@ __device__ double y_image(double ybase, double ypnt, const PGeom& G) {
 // one-period wrap (the box candidate ranges never span more than a period)
 __device__ __forceinline__ int wrap1(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
+__device__ __forceinline__ bool bit_at(const uint32_t* __restrict__ mask, uint32_t n) {
+  return (mask[n >> 5] >> (n & 31u)) & 1u;
+}
+
+// ---- pair-search cells ------------------------------------------------------
+// Cell of coordinate v in [0, n) on an axis of nc cells (each at least as
+// wide as the longest pair range, so partners sit in adjacent cells).
+__device__ __forceinline__ int cell_of(double v, int n, int nc) {
+  int c = (int)(v * (double)nc / (double)n);
+  return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
+}
+
+__device__ __forceinline__ void cell_coords(const double* __restrict__ ypglb, int p, const PGeom& G,
+                                            const CellArgs& C, int& cx_, int& cy_, int& cz_) {
+  cx_ = cell_of(ypglb[3 * p], G.nx, C.ncx);
+  cy_ = cell_of(ypglb[3 * p + 1], G.ny, C.ncy);
+  cz_ = cell_of(ypglb[3 * p + 2], G.nz, C.ncz);
+}
+
+// Index of neighbour cell k (0..26) of (cx_, cy_, cz_), or -1 beyond a wall.
+__device__ __forceinline__ int cell_neighbour(const CellArgs& C, const PGeom& G, int cx_, int cy_,
+                                              int cz_, int k) {
+  const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
+  int y = cy_ + dy;
+  if (G.walls_y) {
+    if (y < 0 || y >= C.ncy) return -1;
+  } else {
+    y = wrap1(y, C.ncy);
+  }
+  const int x = wrap1(cx_ + dx, C.ncx), z = wrap1(cz_ + dz, C.ncz);
+  return x + C.ncx * (y + C.ncy * z);
+}
+
+__global__ void k_cell_zero(int32_t* __restrict__ cnt, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) cnt[i] = 0;
+}
+
+__global__ void __launch_bounds__(256)
+    k_cell_build(const double* __restrict__ ypglb, int np, PGeom G, CellArgs C,
+                 int32_t* __restrict__ status) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  if (!(sane_coord(ypglb[3 * p], G.nx) && sane_coord(ypglb[3 * p + 1], G.ny) &&
+        sane_coord(ypglb[3 * p + 2], G.nz)))
+    return;   // reported by the move / links kernels
+  int a, b, c;
+  cell_coords(ypglb, p, G, C, a, b, c);
+  const int cell = a + C.ncx * (b + C.ncy * c);
+  const int slot = atomicAdd(&C.cnt[cell], 1);
+  if (slot < kCellCap)
+    C.list[cell * kCellCap + slot] = p;
+  else
+    atomicAdd(&status[ST_CELL_OVERFLOW], 1);
+}
+
+// Overlap test of ovlpchk (partlib.f90:118-171) for the pair (p, q), q > p.
+__device__ __forceinline__ bool pair_overlaps(const double* __restrict__ ypglb, int p, int q,
+                                              const PGeom& G) {
+  const double xc = ypglb[3 * p], yc = ypglb[3 * p + 1], zc = ypglb[3 * p + 2];
+  const double xb = near_image(ypglb[3 * q], xc, G.nx);
+  const double yb = G.walls_y ? ypglb[3 * q + 1] : near_image(ypglb[3 * q + 1], yc, G.ny);
+  const double zb = near_image(ypglb[3 * q + 2], zc, G.nz);
+  const double dist = sqrt((xc - xb) * (xc - xb) + (yc - yb) * (yc - yb) + (zc - zb) * (zc - zb));
+  return dist - 2.0 * G.rad < 0.0;
+}
+
+// Does particle p overlap a later one?  Evaluated by the calling block: all
+// later particles (no cells) or one neighbour cell per thread.
+__device__ bool block_overlap(const double* __restrict__ ypglb, int p, int np, const PGeom& G,
+                              const CellArgs& C) {
+  __shared__ int hit;
+  if (threadIdx.x == 0) hit = 0;
+  __syncthreads();
+  if (C.cnt) {
+    if (threadIdx.x < 27) {
+      int a, b, c;
+      cell_coords(ypglb, p, G, C, a, b, c);
+      const int cell = cell_neighbour(C, G, a, b, c, (int)threadIdx.x);
+      if (cell >= 0) {
+        const int n = min(C.cnt[cell], kCellCap);
+        for (int k = 0; k < n; ++k) {
+          const int q = C.list[cell * kCellCap + k];
+          if (q > p && pair_overlaps(ypglb, p, q, G)) hit = 1;
+        }
+      }
+    }
+  } else {
+    for (int q = p + 1 + threadIdx.x; q < np; q += PB)
+      if (pair_overlaps(ypglb, p, q, G)) hit = 1;
+  }
+  __syncthreads();
+  return hit != 0;
+}
+
 // Per-particle constants of a box scan (shared memory), computed by the whole
 // block: thread 0 sets the centre and candidate ranges, then every thread
 // tests one z or y candidate and the first passing index wins by atomicMin.
@@ -123,25 +218,75 @@ __device__ bool node_geom(const BoxInfo& B, const PGeom& G, int iw, int jw, int 
   return true;
 }
 
+__device__ __forceinline__ int block_exclusive_scan(int v, int* sh, int& total);
+
+// Rows (y, z) of one segment's walk: the x run of each row that can hold
+// nodes within r + 1.5 of the centre (a superset: the node test decides),
+// and the runs' prefix sums.  A segment has at most SLAB * kMaxRowCands rows.
+constexpr int kMaxRowCands = 64;
+static_assert(SLAB * kMaxRowCands <= PB, "one thread per row of a segment");
+struct SegRows {
+  int start[SLAB * kMaxRowCands + 1];   // first flattened index of each row
+  short xa[SLAB * kMaxRowCands];        // first box x index of each row's run
+  int nrows;
+};
+
 // Walk of one segment (z-slab s) of the box in passes of PB candidate nodes,
-// the slab flattened x-fastest so every lane of a pass has a node.  Every
-// thread runs the same number of passes (barriers are allowed in fn);
-// fn(active, iw, jw, kw) is called once per pass.
+// the slab's row runs flattened x-fastest so every lane of a pass has a node.
+// Every thread runs the same number of passes (barriers are allowed in fn);
+// fn(active, iw, jw, kw) is called once per pass, in row-major, x-ascending
+// node order.
 template <typename F>
-__device__ __forceinline__ void walk_segment(const BoxInfo& B, const PGeom& G, int s, F&& fn) {
+__device__ __forceinline__ void walk_segment(const BoxInfo& B, const PGeom& G, int s, SegRows& R,
+                                             int* sh, F&& fn) {
   const int z0 = s * SLAB;
-  const int nzs = min(SLAB, B.az.cnt - z0);
-  const int nxy = B.ax.cnt * B.ay.cnt;
-  const int total = nzs > 0 ? nzs * nxy : 0;
+  const int nzs = max(0, min(SLAB, B.az.cnt - z0));
+  const int nrows = nzs * B.ay.cnt;
+  // x run of each row (a row at most PB: SLAB * kMaxRowCands <= PB)
+  int run = 0, xa = 0;
+  const int r = (int)threadIdx.x;
+  if (r < nrows) {
+    const int tz = r / B.ay.cnt, ty = r - tz * B.ay.cnt;
+    const int kw = wrap1(B.az.lo + z0 + tz, G.nz);
+    const int jw = G.walls_y ? B.ay.lo + ty : wrap1(B.ay.lo + ty, G.ny);
+    if (jw >= 0 && jw < G.ny) {
+      const double zp = (double)(kw + 1) - 0.5;
+      const double zz0 = zp - near_image(B.z0, zp, G.nz);
+      const double yp = (double)(jw + 1) - 0.5;
+      const double yy0 = yp - y_image(kw == B.kfirst ? B.y0 : B.yr, yp, G);
+      const double rem = G.rlink2 - yy0 * yy0 - zz0 * zz0;
+      if (fabs(zz0) <= G.radp2 && fabs(yy0) <= G.radp2 && rem >= 0.0) {
+        // node i (unwrapped) sits at i + 0.5; keep |i + 0.5 - x0| <= sqrt(rem) + 1
+        const double dx = sqrt(rem) + 1.0;
+        const int ilo = max((int)ceil(B.x0 - dx - 0.5), B.ax.lo);
+        const int ihi = min((int)floor(B.x0 + dx - 0.5), B.ax.lo + B.ax.cnt - 1);
+        if (ihi >= ilo) {
+          run = ihi - ilo + 1;
+          xa = ilo - B.ax.lo;
+        }
+      }
+    }
+  }
+  int total;
+  const int st = block_exclusive_scan(run, sh, total);
+  if (r < nrows) {
+    R.start[r] = st;
+    R.xa[r] = (short)xa;
+  }
+  if (r == 0) R.start[nrows] = total;
+  __syncthreads();
   for (int t0 = 0; t0 < total; t0 += PB) {
     const int t = t0 + (int)threadIdx.x;
     const bool act = t < total;
     int iw = 0, jw = 0, kw = 0;
     if (act) {
-      const int tz = t / nxy;
-      const int rem = t - tz * nxy;
-      const int ty = rem / B.ax.cnt;
-      const int tx = rem - ty * B.ax.cnt;
+      int lo = 0, hi = nrows;   // the row holding t: start[lo] <= t < start[lo + 1]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (R.start[mid] <= t) lo = mid; else hi = mid;
+      }
+      const int tz = lo / B.ay.cnt, ty = lo - tz * B.ay.cnt;
+      const int tx = R.xa[lo] + (t - R.start[lo]);
       kw = wrap1(B.az.lo + z0 + tz, G.nz);
       jw = G.walls_y ? B.ay.lo + ty : wrap1(B.ay.lo + ty, G.ny);
       iw = wrap1(B.ax.lo + tx, G.nx);
@@ -280,28 +425,17 @@ __device__ __forceinline__ void links_scan_body(int b_, const double* __restrict
                  uint32_t* __restrict__ sol_list, int32_t* __restrict__ sol_cnt,
                  uint32_t* __restrict__ sh_node, uint32_t* __restrict__ sh_hits,
                  int32_t* __restrict__ sh_cnt, int32_t* __restrict__ lk_cnt, int ovl, int istep,
-                 int32_t* __restrict__ status) {
+                 int32_t* __restrict__ status, CellArgs cells) {
   extern __shared__ uint32_t shell[];
   __shared__ BoxInfo B;
   __shared__ int wt[2][PB / 64][2];
   __shared__ int sh[PB / 64];
-  __shared__ int hit;
   const int b = b_;
   const int p = b / nseg, s = b - p * nseg;
   if (p >= np) return;
   if (ovl && s == 0) {
-    if (threadIdx.x == 0) hit = 0;
-    __syncthreads();
     // ovlpchk: flag p if any later particle is closer than 2r
-    const double xc = ypglb[3 * p], yc = ypglb[3 * p + 1], zc = ypglb[3 * p + 2];
-    for (int q = p + 1 + threadIdx.x; q < np; q += PB) {
-      const double xb = near_image(ypglb[3 * q], xc, G.nx);
-      const double yb = G.walls_y ? ypglb[3 * q + 1] : near_image(ypglb[3 * q + 1], yc, G.ny);
-      const double zb = near_image(ypglb[3 * q + 2], zc, G.nz);
-      const double dist = sqrt((xc - xb) * (xc - xb) + (yc - yb) * (yc - yb) + (zc - zb) * (zc - zb));
-      if (dist - 2.0 * G.rad < 0.0) hit = 1;
-    }
-    __syncthreads();
+    const bool hit = block_overlap(ypglb, p, np, G, cells);
     if (threadIdx.x == 0 && hit) {
       atomicAdd(&status[ST_OVERLAP], 1);
       atomicCAS(&status[ST_OVERLAP_STEP], 0, istep);
@@ -319,7 +453,8 @@ __device__ __forceinline__ void links_scan_body(int b_, const double* __restrict
   }
   uint32_t* sol = sol_list + (size_t)b * seg_cap;
   int parity = 0, nsol = 0, nshell = 0;
-  walk_segment(B, G, s, [&](bool act, int iw, int jw, int kw) {
+  __shared__ SegRows R;
+  walk_segment(B, G, s, R, sh, [&](bool act, int iw, int jw, int kw) {
     LinkGeom L;
     const bool in = act && node_geom(B, G, iw, jw, kw, L) && L.rr0 <= G.rlink2;
     const bool solid = in && (L.rr0 - G.r2 <= 0.0);
@@ -369,8 +504,11 @@ __device__ __forceinline__ void links_scan_body(int b_, const double* __restrict
 // each thread sums a contiguous run, the run totals are scanned in LDS.
 __device__ __forceinline__ void scan_counts_body(int b_, const int32_t* __restrict__ cnt, const int32_t* __restrict__ scnt, int nsegs,
                   int64_t* __restrict__ off, int64_t cap, int32_t* __restrict__ status,
-                  int32_t* __restrict__ fill_count) {
-  if (threadIdx.x == 0) *fill_count = 0;   // the next beads_filling starts an empty list
+                  int32_t* __restrict__ fill_count, int32_t* __restrict__ ct_count) {
+  if (threadIdx.x == 0) {
+    *fill_count = 0;   // the next beads_filling starts an empty list
+    *ct_count = 0;     // the expansion lists this build's contact links
+  }
   __shared__ int64_t part[1024];
   __shared__ int64_t spart[1024];
   const int t = threadIdx.x;
@@ -418,7 +556,8 @@ __device__ __forceinline__ void links_expand_body(int b_, const double* __restri
                    const int32_t* __restrict__ sh_cnt, const int64_t* __restrict__ off, int64_t cap,
                    uint32_t* __restrict__ lk_node, uint8_t* __restrict__ lk_dir,
                    double* __restrict__ lk_alpha, int32_t* __restrict__ status, int nsegs,
-                   FillArgs fa) {
+                   FillArgs fa, const uint32_t* __restrict__ mask, uint64_t* __restrict__ ct_list,
+                   int32_t* __restrict__ ct_count) {
   __shared__ BoxInfo B;
   __shared__ int sh[PB / 64];
   const int b = b_;
@@ -469,6 +608,10 @@ __device__ __forceinline__ void links_expand_body(int b_, const double* __restri
     const double alpha = link_alpha(L, xx1, yy1, zz1, rr1, G.r2);
     if (alpha < 0.0 || alpha > 1.0) ++alpha_bad;
     lk_alpha[l] = alpha;
+    if (bit_at(mask, n)) {   // the node is solid for another sphere: a contact link
+      const int slot = atomicAdd(ct_count, 1);
+      if (slot < cap) ct_list[slot] = ((uint64_t)(uint32_t)p << 40) | (uint64_t)l;
+    }
   }
   if (alpha_bad) atomicAdd(&status[ST_ALPHA_FAULT], alpha_bad);
 }
@@ -477,6 +620,26 @@ __device__ __forceinline__ void links_expand_body(int b_, const double* __restri
 // beads_lubforce (partlib.f90:2211-2292): one workgroup sums the pair forces
 // of particle a; each pair is evaluated as the reference does, from the lower
 // id's side (so both partners see bitwise opposite pair forces).
+// Sphere-wall lubrication (channel beads_lubforce, CHAN/partlib.f90:2316-2355,
+// with its wall axis moved to y): the wall at y = 0 pushes up, the wall at
+// y = ny down.  Added to the y pair sum.
+__device__ __forceinline__ double wall_lub(double yd, const PGeom& G, const LubArgs& L) {
+  const double dist0 = L.mingap_w + G.rad;
+  double s = 0.0;
+  double dist = yd, hgap = dist - G.rad;
+  if (dist < dist0) {
+    const double d1 = hgap < 0.0 ? -hgap : 0.0, t = (dist0 - dist) / L.mingap_w;
+    s += (t * t / L.stf0_w + d1 / L.mingap_w / L.stf1_w) * L.fpc;
+  }
+  dist = (double)G.ny - yd;
+  hgap = dist - G.rad;
+  if (dist < dist0) {
+    const double d1 = hgap < 0.0 ? -hgap : 0.0, t = (dist0 - dist) / L.mingap_w;
+    s -= (t * t / L.stf0_w + d1 / L.mingap_w / L.stf1_w) * L.fpc;
+  }
+  return s;
+}
+
 __device__ void lub_block(int a, const double* __restrict__ ypglb, int np, const PGeom& G,
                           const LubArgs& L) {
   __shared__ double red[3][PB / 64];
@@ -513,117 +676,230 @@ __device__ void lub_block(int a, const double* __restrict__ ypglb, int np, const
     double s = 0.0;
 #pragma unroll
     for (int q = 0; q < PB / 64; ++q) s += red[threadIdx.x][q];
-    if (G.walls_y && threadIdx.x == 1) {
-      // sphere-wall lubrication (channel beads_lubforce with its wall axis
-      // moved to y): the wall at y = 0 pushes up, the wall at y = ny down
-      const double dist0 = L.mingap_w + G.rad;
-      const double yd = ypglb[3 * a + 1];
-      double dist = yd, hgap = dist - G.rad;
-      if (dist < dist0) {
-        const double d1 = hgap < 0.0 ? -hgap : 0.0, t = (dist0 - dist) / L.mingap_w;
-        s += (t * t / L.stf0_w + d1 / L.mingap_w / L.stf1_w) * L.fpc;
-      }
-      dist = (double)G.ny - yd;
-      hgap = dist - G.rad;
-      if (dist < dist0) {
-        const double d1 = hgap < 0.0 ? -hgap : 0.0, t = (dist0 - dist) / L.mingap_w;
-        s -= (t * t / L.stf0_w + d1 / L.mingap_w / L.stf1_w) * L.fpc;
-      }
-    }
+    if (G.walls_y && threadIdx.x == 1) s += wall_lub(ypglb[3 * a + 1], G, L);
     L.flubp[3 * a + threadIdx.x] = s;
   }
 }
 
-__global__ void __launch_bounds__(PB)
-    k_lubforce(const double* __restrict__ ypglb, int np, PGeom G, LubArgs L) {
-  if ((int)blockIdx.x < np) lub_block(blockIdx.x, ypglb, np, G, L);
+// Pair force of the pair (a, b) on a (the reference evaluates each pair from
+// the lower id's side, so both partners get bitwise opposite forces); false
+// if the pair is out of range.
+__device__ __forceinline__ bool lub_pair(const double* __restrict__ ypglb, int a, int b,
+                                         const PGeom& G, const LubArgs& L, double pre, double& fx,
+                                         double& fy, double& fz) {
+  const int lo = a < b ? a : b, hi = a < b ? b : a;
+  const double xc = ypglb[3 * lo], yc = ypglb[3 * lo + 1], zc = ypglb[3 * lo + 2];
+  const double xb = near_image(ypglb[3 * hi], xc, G.nx);
+  const double yb = G.walls_y ? ypglb[3 * hi + 1] : near_image(ypglb[3 * hi + 1], yc, G.ny);
+  const double zb = near_image(ypglb[3 * hi + 2], zc, G.nz);
+  const double dx = xc - xb, dy = yc - yb, dz = zc - zb;
+  const double d2 = dx * dx + dy * dy + dz * dz;
+  if (!(d2 < pre)) return false;
+  const double fpd = lub_fpd(sqrt(d2), G.rad, L.mingap, L.fpc, L.stf0, L.stf1);
+  const double sgn = (a == lo) ? 1.0 : -1.0;
+  fx = sgn * (fpd * dx);
+  fy = sgn * (fpd * dy);
+  fz = sgn * (fpd * dz);
+  return true;
 }
 
-__device__ __forceinline__ bool bit_at(const uint32_t* __restrict__ mask, uint32_t n) {
-  return (mask[n >> 5] >> (n & 31u)) & 1u;
+// beads_lubforce of particle a from the cells: partners in the 27 cells
+// around it, summed in increasing id (a fixed order); then the walls.
+__device__ void lub_cells(int a, const double* __restrict__ ypglb, int np, const PGeom& G,
+                          const LubArgs& L, int32_t* __restrict__ status) {
+  if (a >= np) return;
+  const CellArgs& C = L.cells;
+  const double dist0 = 2.0 * G.rad + L.mingap;
+  const double pre = (dist0 + 1e-6) * (dist0 + 1e-6);
+  int ca, cb, cc;
+  cell_coords(ypglb, a, G, C, ca, cb, cc);
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  int last = -1;
+  for (int guard = 0; guard < 27 * kCellCap; ++guard) {
+    int next = 0x7fffffff;
+    for (int k = 0; k < 27; ++k) {
+      const int cell = cell_neighbour(C, G, ca, cb, cc, k);
+      if (cell < 0) continue;
+      const int n = min(C.cnt[cell], kCellCap);
+      for (int e = 0; e < n; ++e) {
+        const int b = C.list[cell * kCellCap + e];
+        if (b <= last || b >= next || b == a) continue;
+        double fx, fy, fz;
+        if (lub_pair(ypglb, a, b, G, L, pre, fx, fy, fz)) next = b;
+      }
+    }
+    if (next == 0x7fffffff) break;
+    double fx, fy, fz;
+    lub_pair(ypglb, a, next, G, L, pre, fx, fy, fz);
+    sx += fx;
+    sy += fy;
+    sz += fz;
+    last = next;
+  }
+  if (G.walls_y) sy += wall_lub(ypglb[3 * a + 1], G, L);
+  L.flubp[3 * a] = sx;
+  L.flubp[3 * a + 1] = sy;
+  L.flubp[3 * a + 2] = sz;
+}
+
+__global__ void __launch_bounds__(PB)
+    k_lubforce(const double* __restrict__ ypglb, int np, PGeom G, LubArgs L, int32_t* __restrict__ status) {
+  if (L.cells.cnt)
+    lub_cells((int)(blockIdx.x * PB + threadIdx.x), ypglb, np, G, L, status);
+  else if ((int)blockIdx.x < np)
+    lub_block(blockIdx.x, ypglb, np, G, L);
+}
+
+
+// Bounce-back of one link (partlib.f90:1363-1703): the wall point and its
+// rigid-body velocity, then the interpolated value f9 sent back into the
+// fluid and ff1 = f_q(x) post-collision; ff1 + f9 is the link's momentum
+// exchange.  (xx0, yy0, zz0) is the wall point relative to the centre.
+// With `gather` false only the geometry is evaluated.
+template <typename T>
+__device__ __forceinline__ void link_bounce(const T* A, const Grid& g, const PGeom& G,
+                                            const uint32_t* __restrict__ mask, uint32_t n, int q,
+                                            double alpha, double xcp, double ycp, double zcp,
+                                            double w1, double w2, double w3, double o1, double o2,
+                                            double o3, double ww1, double ww2, bool gather,
+                                            double& ff1, double& f9, double& xx0, double& yy0,
+                                            double& zz0) {
+  int x, y, z;
+  uint32_t row;
+  node_coords(n, g, x, y, z, row);
+  const int ix = cx(q), iy = cy(q), iz = cz(q);
+  const double xp = (double)(x + 1) - 0.5, yp = (double)(y + 1) - 0.5, zp = (double)(z + 1) - 0.5;
+  const double xc = near_image(xcp, xp, g.nx);
+  const double yc = G.walls_y ? ycp : near_image(ycp, yp, g.ny);
+  const double zc = near_image(zcp, zp, g.nz);
+  xx0 = xp + (double)ix * alpha - xc;
+  yy0 = yp + (double)iy * alpha - yc;
+  zz0 = zp + (double)iz * alpha - zc;
+  if (!gather) return;
+  const double uwx = w1 + o2 * zz0 - o3 * yy0;
+  const double uwy = w2 + o3 * xx0 - o1 * zz0;
+  const double uwz = w3 + o1 * yy0 - o2 * xx0;
+  const double uwpro = uwx * (double)ix + uwy * (double)iy + uwz * (double)iz;
+  // x - c and x - 2c (periodic; beyond a wall counts as solid)
+  const int y1 = y - iy, y2 = y - 2 * iy;
+  const bool out1 = g.walls_y && (y1 < 0 || y1 >= g.ny);
+  const bool out2 = g.walls_y && (y2 < 0 || y2 >= g.ny);
+  const uint32_t m1 = out1 ? n : (uint32_t)wrap1(x - ix, g.nx) + (uint32_t)g.nx * ((uint32_t)wrap1(y1, g.ny) + (uint32_t)g.ny * (uint32_t)wrap1(z - iz, g.nz));
+  const uint32_t m2 = out2 ? n : (uint32_t)wrap1(x - 2 * ix, g.nx) + (uint32_t)g.nx * ((uint32_t)wrap1(y2, g.ny) + (uint32_t)g.ny * (uint32_t)wrap1(z - 2 * iz, g.nz));
+  // only the values the interpolation uses are gathered: simple bounce-back
+  // needs f_q(x); the alpha <= 1/2 branch f_q at x - c and x - 2c, the
+  // alpha > 1/2 branch f_qbar at x and x - c
+  const bool sol1 = out1 || bit_at(mask, m1);
+  const int qo = opp(q);
+  const double wt = q <= 6 ? ww1 : ww2;
+  ff1 = post_value(A, g, q, n);
+  if (alpha <= 0.5) {
+    bool sol2 = true;
+    double ff2a = 0.0, ff3a = 0.0;
+    if (!sol1) {
+      sol2 = out2 || bit_at(mask, m2);
+      ff2a = post_value(A, g, q, m1);
+      if (!sol2) ff3a = post_value(A, g, q, m2);
+    }
+    f9 = bounce_back_value(alpha, ff1, ff2a, ff3a, sol1, sol2, 0.0, 0.0, wt, uwpro);
+  } else {
+    const double ffo = post_value(A, g, qo, n);
+    const double ffo1 = sol1 ? 0.0 : post_value(A, g, qo, m1);
+    f9 = bounce_back_value(alpha, ff1, 0.0, 0.0, sol1, true, ffo, ffo1, wt, uwpro);
+  }
+}
+
+// Contact links (node x solid for another sphere, spheres almost touching)
+// read post-collision values at a solid node, which the bounce-back grid
+// overwrites: their values are taken first, from the untouched field, into
+// lk_out (f9, ff1).  Every other link reads fluid nodes only and writes only
+// solid ones, so the bounce-back grid can scatter as it goes; all links see
+// the one post-collision snapshot.  Grid-stride over the contact list.
+template <typename T>
+__device__ __forceinline__ void bb_contact_body(int b_, int nblk, const BounceArgs& a) {
+  const T* A = (const T*)a.A;
+  const int64_t total = min((int64_t)*a.ct_count, a.cap);
+  for (int64_t e = (int64_t)b_ * PB + threadIdx.x; e < total; e += (int64_t)nblk * PB) {
+    const uint64_t ent = a.ct_list[e];
+    const int p = (int)(ent >> 40);
+    const int64_t l = (int64_t)(ent & ((1ull << 40) - 1));
+    if (p >= a.np || l >= a.cap) continue;   // never true for a consistent list
+    const uint32_t n = a.lk_node[l];
+    const int q = a.lk_dir[l];
+    if (n >= a.g.ncell || q < 1 || q >= NPOP) continue;
+    double ff1 = 0.0, f9 = 0.0, xx0, yy0, zz0;
+    link_bounce<T>(A, a.g, a.G, a.mask, n, q, a.lk_alpha[l], a.ypglb[3 * p], a.ypglb[3 * p + 1],
+                   a.ypglb[3 * p + 2], a.wp[3 * p], a.wp[3 * p + 1], a.wp[3 * p + 2], a.omgp[3 * p],
+                   a.omgp[3 * p + 1], a.omgp[3 * p + 2], a.ww1, a.ww2, true, ff1, f9, xx0, yy0, zz0);
+    a.lk_out[l] = make_double2(f9, ff1);
+  }
 }
 
 // beads_collision (partlib.f90:1363-1703), one segment of links per block:
-// the interpolated bounce-back value of every link into lk_out, and the
+// the bounce-back value of every link, written at once into the opposite
+// population of the solid neighbour x + c (partlib.f90:1605-1621), and the
 // segment's momentum-exchange force and torque (FP64, fixed reduction order)
-// into seg_ft.  All links read one snapshot of the post-collision field; the
-// values are scattered into the solid nodes by k_bb_scatter afterwards.
+// into seg_ft.  Blocks past the link segments compute beads_lubforce of one
+// particle, then (fused step) reset one segment of the map buffer the coming
+// beads_links rebuilds (built two steps ago).
 template <typename T>
-__device__ __forceinline__ void bounce_back_body(int b_, const T* __restrict__ A, Grid g, PGeom G, const uint32_t* __restrict__ mask,
-                  int nseg, const int64_t* __restrict__ off, int64_t cap,
-                  const uint32_t* __restrict__ lk_node, const uint8_t* __restrict__ lk_dir,
-                  const double* __restrict__ lk_alpha, double* __restrict__ lk_out,
-                  const double* __restrict__ ypglb, const double* __restrict__ wp,
-                  const double* __restrict__ omgp, double ww1, double ww2,
-                  double* __restrict__ seg_ft, int32_t* __restrict__ status, int nsegs, int np,
-                  LubArgs lub) {
+__device__ __forceinline__ void bounce_back_body(int b_, const BounceArgs& a) {
   __shared__ double red[6][PB / 64];
+  T* A = (T*)a.A;
+  const Grid& g = a.g;
   const int b = b_;
-  if (b >= nsegs) {
-    // blocks past the link segments: beads_lubforce of one particle
-    lub_block(b - nsegs, ypglb, np, G, lub);
+  if (b >= a.nsegs) {
+    const int r = b - a.nsegs;
+    if (r < a.nlub) {
+      if (a.lub.cells.cnt)
+        lub_cells(r * PB + (int)threadIdx.x, a.ypglb, a.np, a.G, a.lub, a.status);
+      else
+        lub_block(r, a.ypglb, a.np, a.G, a.lub);
+    } else
+      clear_segment(r - a.nlub, a.nseg, a.seg_cap, a.clr_list, a.clr_cnt, a.clr_isn, a.clr_mask);
     return;
   }
-  const int p = b / nseg;
-  const int64_t a = off[b];
-  const int64_t e = min(off[b + 1], cap);
-  const double xcp = ypglb[3 * p], ycp = ypglb[3 * p + 1], zcp = ypglb[3 * p + 2];
-  const double w1 = wp[3 * p], w2 = wp[3 * p + 1], w3 = wp[3 * p + 2];
-  const double o1 = omgp[3 * p], o2 = omgp[3 * p + 1], o3 = omgp[3 * p + 2];
+  const int p = b / a.nseg;
+  const int64_t l0 = a.off[b];
+  const int64_t e = min(a.off[b + 1], a.cap);
+  const double xcp = a.ypglb[3 * p], ycp = a.ypglb[3 * p + 1], zcp = a.ypglb[3 * p + 2];
+  const double w1 = a.wp[3 * p], w2 = a.wp[3 * p + 1], w3 = a.wp[3 * p + 2];
+  const double o1 = a.omgp[3 * p], o2 = a.omgp[3 * p + 1], o3 = a.omgp[3 * p + 2];
   double acc[6] = {0, 0, 0, 0, 0, 0};
   int contact = 0;
-  for (int64_t l = a + threadIdx.x; l < e; l += PB) {
-    const uint32_t n = lk_node[l];
-    const int q = lk_dir[l];
-    const double alpha = lk_alpha[l];
+  for (int64_t l = l0 + threadIdx.x; l < e; l += PB) {
+    const uint32_t n = a.lk_node[l];
+    const int q = a.lk_dir[l];
+    const double alpha = a.lk_alpha[l];
     if (n >= g.ncell || q < 1 || q >= NPOP) continue;  // never true for a consistent list
+    const bool cont = bit_at(a.mask, n);
+    double ff1 = 0.0, f9 = 0.0, xx0, yy0, zz0;
+    link_bounce<T>(A, g, a.G, a.mask, n, q, alpha, xcp, ycp, zcp, w1, w2, w3, o1, o2, o3, a.ww1,
+                   a.ww2, !cont, ff1, f9, xx0, yy0, zz0);
+    if (cont) {
+      const double2 v = a.lk_out[l];
+      f9 = v.x;
+      ff1 = v.y;
+      ++contact;
+    }
+    // scatter: the post-collision value of opp(q) at the solid node x + c
     int x, y, z;
     uint32_t row;
     node_coords(n, g, x, y, z, row);
-    const int ix = cx(q), iy = cy(q), iz = cz(q);
-    const double xp = (double)(x + 1) - 0.5, yp = (double)(y + 1) - 0.5, zp = (double)(z + 1) - 0.5;
-    const double xc = near_image(xcp, xp, g.nx);
-    const double yc = G.walls_y ? ycp : near_image(ycp, yp, g.ny);
-    const double zc = near_image(zcp, zp, g.nz);
-    const double xx0 = xp + (double)ix * alpha - xc;
-    const double yy0 = yp + (double)iy * alpha - yc;
-    const double zz0 = zp + (double)iz * alpha - zc;
-    const double uwx = w1 + o2 * zz0 - o3 * yy0;
-    const double uwy = w2 + o3 * xx0 - o1 * zz0;
-    const double uwz = w3 + o1 * yy0 - o2 * xx0;
-    const double uwpro = uwx * (double)ix + uwy * (double)iy + uwz * (double)iz;
-    // x - c and x - 2c (periodic; beyond a wall counts as solid)
-    const int y1 = y - iy, y2 = y - 2 * iy;
-    const bool out1 = g.walls_y && (y1 < 0 || y1 >= g.ny);
-    const bool out2 = g.walls_y && (y2 < 0 || y2 >= g.ny);
-    const uint32_t m1 = out1 ? n : (uint32_t)wrap1(x - ix, g.nx) + (uint32_t)g.nx * ((uint32_t)wrap1(y1, g.ny) + (uint32_t)g.ny * (uint32_t)wrap1(z - iz, g.nz));
-    const uint32_t m2 = out2 ? n : (uint32_t)wrap1(x - 2 * ix, g.nx) + (uint32_t)g.nx * ((uint32_t)wrap1(y2, g.ny) + (uint32_t)g.ny * (uint32_t)wrap1(z - 2 * iz, g.nz));
-    // only the values the interpolation uses are gathered: simple bounce-back
-    // needs f_q(x); the alpha <= 1/2 branch f_q at x - c and x - 2c, the
-    // alpha > 1/2 branch f_qbar at x and x - c
-    const bool sol1 = out1 || bit_at(mask, m1);
-    const int qo = opp(q);
-    const double wt = q <= 6 ? ww1 : ww2;
-    const double ff1 = post_value(A, g, q, n);
-    double f9;
-    if (alpha <= 0.5) {
-      bool sol2 = true;
-      double ff2a = 0.0, ff3a = 0.0;
-      if (!sol1) {
-        sol2 = out2 || bit_at(mask, m2);
-        ff2a = post_value(A, g, q, m1);
-        if (!sol2) ff3a = post_value(A, g, q, m2);
-      }
-      f9 = bounce_back_value(alpha, ff1, ff2a, ff3a, sol1, sol2, 0.0, 0.0, wt, uwpro);
-    } else {
-      const double ffo = post_value(A, g, qo, n);
-      const double ffo1 = sol1 ? 0.0 : post_value(A, g, qo, m1);
-      f9 = bounce_back_value(alpha, ff1, 0.0, 0.0, sol1, true, ffo, ffo1, wt, uwpro);
+    const int yt = y + cy(q);
+    if (!(g.walls_y && (yt < 0 || yt >= g.ny))) {
+      const int xt = wrap1(x + cx(q), g.nx), zt = wrap1(z + cz(q), g.nz), ytw = wrap1(yt, g.ny);
+      const uint32_t t = (uint32_t)xt + (uint32_t)g.nx * ((uint32_t)ytw + (uint32_t)g.ny * (uint32_t)zt);
+      int pl;
+      uint32_t s;
+      if (g.walls_y)
+        post_src<true>(g.lay, opp(q), xt, ytw, zt, t, g, pl, s);
+      else
+        post_src<false>(g.lay, opp(q), xt, ytw, zt, t, g, pl, s);
+      A[(uint64_t)pl * g.ncell + s] = (T)f9;
     }
-    lk_out[l] = f9;
-    if (bit_at(mask, n)) ++contact;
     const double dff = ff1 + f9;
-    const double dxm = dff * (double)ix, dym = dff * (double)iy, dzm = dff * (double)iz;
+    const double dxm = dff * (double)cx(q), dym = dff * (double)cy(q), dzm = dff * (double)cz(q);
     acc[0] += dxm;
     acc[1] += dym;
     acc[2] += dzm;
@@ -645,58 +921,37 @@ __device__ __forceinline__ void bounce_back_body(int b_, const T* __restrict__ A
     double v = 0.0;
 #pragma unroll
     for (int q = 0; q < PB / 64; ++q) v += red[threadIdx.x][q];
-    seg_ft[6ll * b + threadIdx.x] = v;
+    a.seg_ft[6ll * b + threadIdx.x] = v;
   }
-  if (contact) atomicAdd(&status[ST_CONTACT_LINKS], contact);
+  if (contact) atomicAdd(&a.status[ST_CONTACT_LINKS], contact);
 }
 
-// Write each link's bounce-back value into the opposite population of the
-// solid neighbour x + c (partlib.f90:1605-1621); the first segment of each
-// particle also sums its segments' force and torque into fHIp / torqp.
-template <typename T>
-__device__ __forceinline__ void bb_scatter_body(int b_, T* __restrict__ A, Grid g, int nseg, const int64_t* __restrict__ off, int64_t cap,
-                 const uint32_t* __restrict__ lk_node, const uint8_t* __restrict__ lk_dir,
-                 const double* __restrict__ lk_out, const double* __restrict__ seg_ft,
-                 double* __restrict__ fHIp, double* __restrict__ torqp, int nsegs, int seg_cap,
-                 const uint32_t* __restrict__ clr_list, const int32_t* __restrict__ clr_cnt,
-                 int32_t* __restrict__ clr_isn, uint32_t* __restrict__ clr_mask) {
-  const int b = b_;
-  if (b >= nsegs) {
-    // blocks past the link segments: reset one segment of the map buffer the
-    // coming beads_links rebuilds (it was built two steps ago)
-    clear_segment(b - nsegs, nseg, seg_cap, clr_list, clr_cnt, clr_isn, clr_mask);
-    return;
+// fHIp / torqp of each particle: its segments' sums, in segment order.
+__device__ __forceinline__ void seg_sum(const double* __restrict__ seg_ft, int nseg, int p,
+                                        double (&f)[3], double (&t)[3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double v = 0.0, u = 0.0;
+    for (int s = 0; s < nseg; ++s) {
+      v += seg_ft[6ll * (p * nseg + s) + c];
+      u += seg_ft[6ll * (p * nseg + s) + 3 + c];
+    }
+    f[c] = v;
+    t[c] = u;
   }
-  const int p = b / nseg;
-  if (b - p * nseg == 0 && threadIdx.x < 6) {
-    double v = 0.0;
-    for (int s = 0; s < nseg; ++s) v += seg_ft[6ll * (b + s) + threadIdx.x];
-    if (threadIdx.x < 3)
-      fHIp[3 * p + threadIdx.x] = v;
-    else
-      torqp[3 * p + threadIdx.x - 3] = v;
-  }
-  const int64_t a = off[b];
-  const int64_t e = min(off[b + 1], cap);
-  for (int64_t l = a + threadIdx.x; l < e; l += PB) {
-    const uint32_t n = lk_node[l];
-    const int q = lk_dir[l];
-    if (n >= g.ncell || q < 1 || q >= NPOP) continue;
-    int x, y, z;
-    uint32_t row;
-    node_coords(n, g, x, y, z, row);
-    const int yt = y + cy(q);
-    if (g.walls_y && (yt < 0 || yt >= g.ny)) continue;
-    const int xt = wrap1(x + cx(q), g.nx), zt = wrap1(z + cz(q), g.nz);
-    const uint32_t t = (uint32_t)xt + (uint32_t)g.nx * ((uint32_t)wrap1(yt, g.ny) + (uint32_t)g.ny * (uint32_t)zt);
-    // the post-collision value of opp(q) at the solid node t
-    int pl;
-    uint32_t s;
-    if (g.walls_y)
-      post_src<true>(g.lay, opp(q), xt, wrap1(yt, g.ny), zt, t, g, pl, s);
-    else
-      post_src<false>(g.lay, opp(q), xt, wrap1(yt, g.ny), zt, t, g, pl, s);
-    A[(uint64_t)pl * g.ncell + s] = (T)lk_out[l];
+}
+
+__global__ void __launch_bounds__(256)
+    k_seg_reduce(const double* __restrict__ seg_ft, int nseg, int np, double* __restrict__ fHIp,
+                 double* __restrict__ torqp) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  double f[3], t[3];
+  seg_sum(seg_ft, nseg, p, f, t);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    fHIp[3 * p + c] = f[c];
+    torqp[3 * p + c] = t[c];
   }
 }
 
@@ -721,9 +976,13 @@ __device__ __forceinline__ bool wrap_particle(double* yp, double* ypglb, double*
 
 // beads_move (partlib.f90:2295-2394): second-order Newton-Euler update.  With
 // `wrap` set the periodic wrap of beads_redistribute (partlib.f90:2600-2626)
-// follows in the same thread.  Every input is loaded before the first store
-// (the argument arrays are distinct, but the compiler cannot know it).
-__device__ __forceinline__ void move_body(int b_, MoveArgs m, int np, int first, int wrap, PGeom G, int32_t* __restrict__ status) {
+// follows in the same thread; with seg_ft set the hydrodynamic force and
+// torque are first summed from the bounce-back segments (and stored).  Every
+// input is loaded before the first store (the argument arrays are distinct,
+// but the compiler cannot know it).
+__device__ __forceinline__ void move_body(int b_, MoveArgs m, int np, int first, int wrap, PGeom G,
+                                          int32_t* __restrict__ status,
+                                          const double* __restrict__ seg_ft, int nseg) {
   const int p = b_ * blockDim.x + threadIdx.x;
   if (p >= np) return;
   const double dt = 1.0;
@@ -738,15 +997,18 @@ __device__ __forceinline__ void move_body(int b_, MoveArgs m, int np, int first,
     yg[c] = m.ypglb[q];
     wp[c] = m.wp[q];
     omgp[c] = m.omgp[q];
-    fhi[c] = m.fHIp[q];
+    if (!seg_ft) {
+      fhi[c] = m.fHIp[q];
+      torq[c] = m.torqp[q];
+    }
     flub[c] = m.flubp[q];
-    torq[c] = m.torqp[q];
     fpp[c] = m.forcepp[q];
     tpp[c] = m.torqpp[q];
     dwdt[c] = m.dwdt[q];
     domgdt[c] = m.domgdt[q];
     th[c] = first ? 0.0 : m.thetap[q];
   }
+  if (seg_ft) seg_sum(seg_ft, nseg, p, fhi, torq);
   double wpp[3], omgpp[3], fp[3], dwn[3], don[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -806,6 +1068,10 @@ __device__ __forceinline__ void move_body(int b_, MoveArgs m, int np, int first,
     m.domgdt[q] = don[c];
     m.forcepp[q] = fp[c];
     m.torqpp[q] = torq[c];
+    if (seg_ft) {
+      m.fHIp[q] = fhi[c];
+      m.torqp[q] = torq[c];
+    }
     if (wrap) {
       m.ypglbp[q] = ygp[c];
       m.ypglb[q] = yg[c];
@@ -827,22 +1093,11 @@ __global__ void __launch_bounds__(256)
 // ovlpchk (partlib.f90:118-171): one workgroup per particle flags it if any
 // later particle is closer than 2r.
 __global__ void __launch_bounds__(PB)
-    k_overlap(const double* __restrict__ ypglb, int np, PGeom G, int istep,
+    k_overlap(const double* __restrict__ ypglb, int np, PGeom G, int istep, CellArgs cells,
               int32_t* __restrict__ status) {
-  __shared__ int hit;
   const int a = blockIdx.x;
   if (a >= np) return;
-  if (threadIdx.x == 0) hit = 0;
-  __syncthreads();
-  const double xc = ypglb[3 * a], yc = ypglb[3 * a + 1], zc = ypglb[3 * a + 2];
-  for (int b = a + 1 + threadIdx.x; b < np; b += PB) {
-    const double xb = near_image(ypglb[3 * b], xc, G.nx);
-    const double yb = G.walls_y ? ypglb[3 * b + 1] : near_image(ypglb[3 * b + 1], yc, G.ny);
-    const double zb = near_image(ypglb[3 * b + 2], zc, G.nz);
-    const double dist = sqrt((xc - xb) * (xc - xb) + (yc - yb) * (yc - yb) + (zc - zb) * (zc - zb));
-    if (dist - 2.0 * G.rad < 0.0) hit = 1;
-  }
-  __syncthreads();
+  const bool hit = block_overlap(ypglb, a, np, G, cells);
   if (threadIdx.x == 0 && hit) {
     atomicAdd(&status[ST_OVERLAP], 1);
     atomicCAS(&status[ST_OVERLAP_STEP], 0, istep);
@@ -996,67 +1251,67 @@ __device__ __forceinline__ int case_of(const int* __restrict__ off, int nc, int 
 }
 
 __global__ void __launch_bounds__(PB) k_links_scan(ScanArgs a) {
-  links_scan_body((int)blockIdx.x, a.ypglb, a.np, a.nseg, a.seg_cap, a.G, a.isn, a.mask, a.sol_list, a.sol_cnt, a.sh_node, a.sh_hits, a.sh_cnt, a.lk_cnt, a.ovl, a.istep, a.status);
+  links_scan_body((int)blockIdx.x, a.ypglb, a.np, a.nseg, a.seg_cap, a.G, a.isn, a.mask, a.sol_list, a.sol_cnt, a.sh_node, a.sh_hits, a.sh_cnt, a.lk_cnt, a.ovl, a.istep, a.status, a.cells);
 }
 __global__ void __launch_bounds__(PB)
     kb_links_scan(const ScanArgs* __restrict__ t, const int* __restrict__ off, int nc) {
   const int c = case_of(off, nc, (int)blockIdx.x);
   const ScanArgs a = t[c];
-  links_scan_body((int)blockIdx.x - off[c], a.ypglb, a.np, a.nseg, a.seg_cap, a.G, a.isn, a.mask, a.sol_list, a.sol_cnt, a.sh_node, a.sh_hits, a.sh_cnt, a.lk_cnt, a.ovl, a.istep, a.status);
+  links_scan_body((int)blockIdx.x - off[c], a.ypglb, a.np, a.nseg, a.seg_cap, a.G, a.isn, a.mask, a.sol_list, a.sol_cnt, a.sh_node, a.sh_hits, a.sh_cnt, a.lk_cnt, a.ovl, a.istep, a.status, a.cells);
 }
 
 __global__ void __launch_bounds__(1024) k_scan_counts(ScanCountArgs a) {
-  scan_counts_body((int)blockIdx.x, a.cnt, a.scnt, a.nsegs, a.off, a.cap, a.status, a.fill_count);
+  scan_counts_body((int)blockIdx.x, a.cnt, a.scnt, a.nsegs, a.off, a.cap, a.status, a.fill_count, a.ct_count);
 }
 __global__ void __launch_bounds__(1024)
     kb_scan_counts(const ScanCountArgs* __restrict__ t, const int* __restrict__ off, int nc) {
   const int c = case_of(off, nc, (int)blockIdx.x);
   const ScanCountArgs a = t[c];
-  scan_counts_body((int)blockIdx.x - off[c], a.cnt, a.scnt, a.nsegs, a.off, a.cap, a.status, a.fill_count);
+  scan_counts_body((int)blockIdx.x - off[c], a.cnt, a.scnt, a.nsegs, a.off, a.cap, a.status, a.fill_count, a.ct_count);
 }
 
 __global__ void __launch_bounds__(PB) k_links_expand(ExpandArgs a) {
-  links_expand_body((int)blockIdx.x, a.ypglb, a.np, a.nseg, a.seg_cap, a.G, a.sh_node, a.sh_hits, a.sh_cnt, a.off, a.cap, a.lk_node, a.lk_dir, a.lk_alpha, a.status, a.nsegs, a.fa);
+  links_expand_body((int)blockIdx.x, a.ypglb, a.np, a.nseg, a.seg_cap, a.G, a.sh_node, a.sh_hits, a.sh_cnt, a.off, a.cap, a.lk_node, a.lk_dir, a.lk_alpha, a.status, a.nsegs, a.fa, a.mask, a.ct_list, a.ct_count);
 }
 __global__ void __launch_bounds__(PB)
     kb_links_expand(const ExpandArgs* __restrict__ t, const int* __restrict__ off, int nc) {
   const int c = case_of(off, nc, (int)blockIdx.x);
   const ExpandArgs a = t[c];
-  links_expand_body((int)blockIdx.x - off[c], a.ypglb, a.np, a.nseg, a.seg_cap, a.G, a.sh_node, a.sh_hits, a.sh_cnt, a.off, a.cap, a.lk_node, a.lk_dir, a.lk_alpha, a.status, a.nsegs, a.fa);
+  links_expand_body((int)blockIdx.x - off[c], a.ypglb, a.np, a.nseg, a.seg_cap, a.G, a.sh_node, a.sh_hits, a.sh_cnt, a.off, a.cap, a.lk_node, a.lk_dir, a.lk_alpha, a.status, a.nsegs, a.fa, a.mask, a.ct_list, a.ct_count);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(PB) k_bb_contact(BounceArgs a, int nblk) {
+  bb_contact_body<T>((int)blockIdx.x, nblk, a);
+}
+template <typename T>
+__global__ void __launch_bounds__(PB)
+    kb_bb_contact(const BounceArgs* __restrict__ t, const int* __restrict__ off, int nc) {
+  const int c = case_of(off, nc, (int)blockIdx.x);
+  const BounceArgs a = t[c];
+  bb_contact_body<T>((int)blockIdx.x - off[c], off[c + 1] - off[c], a);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(PB) k_bounce_back(BounceArgs a) {
-  bounce_back_body<T>((int)blockIdx.x, (const T*)a.A, a.g, a.G, a.mask, a.nseg, a.off, a.cap, a.lk_node, a.lk_dir, a.lk_alpha, a.lk_out, a.ypglb, a.wp, a.omgp, a.ww1, a.ww2, a.seg_ft, a.status, a.nsegs, a.np, a.lub);
+  bounce_back_body<T>((int)blockIdx.x, a);
 }
 template <typename T>
 __global__ void __launch_bounds__(PB)
     kb_bounce_back(const BounceArgs* __restrict__ t, const int* __restrict__ off, int nc) {
   const int c = case_of(off, nc, (int)blockIdx.x);
   const BounceArgs a = t[c];
-  bounce_back_body<T>((int)blockIdx.x - off[c], (const T*)a.A, a.g, a.G, a.mask, a.nseg, a.off, a.cap, a.lk_node, a.lk_dir, a.lk_alpha, a.lk_out, a.ypglb, a.wp, a.omgp, a.ww1, a.ww2, a.seg_ft, a.status, a.nsegs, a.np, a.lub);
-}
-
-template <typename T>
-__global__ void __launch_bounds__(PB) k_bb_scatter(ScatterArgs a) {
-  bb_scatter_body<T>((int)blockIdx.x, (T*)a.A, a.g, a.nseg, a.off, a.cap, a.lk_node, a.lk_dir, a.lk_out, a.seg_ft, a.fHIp, a.torqp, a.nsegs, a.seg_cap, a.clr_list, a.clr_cnt, a.clr_isn, a.clr_mask);
-}
-template <typename T>
-__global__ void __launch_bounds__(PB)
-    kb_bb_scatter(const ScatterArgs* __restrict__ t, const int* __restrict__ off, int nc) {
-  const int c = case_of(off, nc, (int)blockIdx.x);
-  const ScatterArgs a = t[c];
-  bb_scatter_body<T>((int)blockIdx.x - off[c], (T*)a.A, a.g, a.nseg, a.off, a.cap, a.lk_node, a.lk_dir, a.lk_out, a.seg_ft, a.fHIp, a.torqp, a.nsegs, a.seg_cap, a.clr_list, a.clr_cnt, a.clr_isn, a.clr_mask);
+  bounce_back_body<T>((int)blockIdx.x - off[c], a);
 }
 
 __global__ void __launch_bounds__(256) k_move(MoveKArgs a) {
-  move_body((int)blockIdx.x, a.m, a.np, a.first, a.wrap, a.G, a.status);
+  move_body((int)blockIdx.x, a.m, a.np, a.first, a.wrap, a.G, a.status, a.seg_ft, a.nseg);
 }
 __global__ void __launch_bounds__(256)
     kb_move(const MoveKArgs* __restrict__ t, const int* __restrict__ off, int nc) {
   const int c = case_of(off, nc, (int)blockIdx.x);
   const MoveKArgs a = t[c];
-  move_body((int)blockIdx.x - off[c], a.m, a.np, a.first, a.wrap, a.G, a.status);
+  move_body((int)blockIdx.x - off[c], a.m, a.np, a.first, a.wrap, a.G, a.status, a.seg_ft, a.nseg);
 }
 
 template <typename T, bool WALLS>
@@ -1181,6 +1436,36 @@ static bool make_fill_args(Ctx& c, int old, int neu, FillArgs& fa) {
   return true;
 }
 
+// The cell grid if it is on (else cnt = nullptr: all pairs).  The lists must
+// be current (ensure_cells) before a kernel reads them.
+static CellArgs cell_args(const Ctx& c) {
+  const Particles& P = c.pt;
+  CellArgs C;
+  C.cnt = c.ens ? nullptr : P.cell_cnt;   // ensemble members search all pairs
+  C.list = P.cell_list;
+  C.ncx = P.cell_n[0];
+  C.ncy = P.cell_n[1];
+  C.ncz = P.cell_n[2];
+  return C;
+}
+
+// Blocks of the lubrication part of a grid: one per particle (all pairs) or
+// one thread per particle (cells).
+static int lub_blocks(const Ctx& c) {
+  return cell_args(c).cnt ? (c.pt.np + PB - 1) / PB : c.pt.np;
+}
+
+hipError_t ensure_cells(Ctx& c) {
+  Particles& P = c.pt;
+  if (!cell_args(c).cnt || P.cells_ok) return hipSuccess;
+  const int ncell = P.cell_n[0] * P.cell_n[1] * P.cell_n[2];
+  hipLaunchKernelGGL(k_cell_zero, dim3((ncell + 255) / 256), dim3(256), 0, c.stream, P.cell_cnt, ncell);
+  hipLaunchKernelGGL(k_cell_build, dim3((P.np + 255) / 256), dim3(256), 0, c.stream, P.ypglb, P.np,
+                     pgeom(c), cell_args(c), c.d_status);
+  P.cells_ok = true;
+  return hipGetLastError();
+}
+
 static LubArgs lub_args(Ctx& c) {
   LubArgs L;
   L.mingap = c.p.mingap;
@@ -1191,6 +1476,7 @@ static LubArgs lub_args(Ctx& c) {
   L.stf0_w = c.p.stf0_w;
   L.stf1_w = c.p.stf1_w;
   L.flubp = c.pt.flubp;
+  L.cells = cell_args(c);
   return L;
 }
 
@@ -1204,6 +1490,7 @@ ScanArgs scan_args(Ctx& c, int target, int ovl, int istep) {
   a.isn = P.isn[target]; a.mask = P.mask[target]; a.sol_list = P.sol_list[target];
   a.sol_cnt = P.sol_cnt[target]; a.sh_node = P.sh_node; a.sh_hits = P.sh_hits; a.sh_cnt = P.sh_cnt;
   a.lk_cnt = P.lk_cnt; a.ovl = ovl; a.istep = istep; a.status = c.d_status;
+  a.cells = cell_args(c);
   return a;
 }
 
@@ -1211,7 +1498,7 @@ ScanCountArgs count_args(Ctx& c, int target) {
   Particles& P = c.pt;
   ScanCountArgs a;
   a.cnt = P.lk_cnt; a.scnt = P.sol_cnt[target]; a.nsegs = P.np * P.nseg; a.off = P.lk_off;
-  a.cap = P.lk_cap; a.status = c.d_status; a.fill_count = P.fill_count;
+  a.cap = P.lk_cap; a.status = c.d_status; a.fill_count = P.fill_count; a.ct_count = P.ct_count;
   return a;
 }
 
@@ -1224,6 +1511,7 @@ ExpandArgs expand_args(Ctx& c, int target, bool with_find, int* blocks) {
   a.sh_node = P.sh_node; a.sh_hits = P.sh_hits; a.sh_cnt = P.sh_cnt; a.off = P.lk_off; a.cap = P.lk_cap;
   a.lk_node = P.lk_node; a.lk_dir = P.lk_dir; a.lk_alpha = P.lk_alpha; a.status = c.d_status;
   a.nsegs = P.np * P.nseg;
+  a.mask = P.mask[target]; a.ct_list = P.ct_list; a.ct_count = P.ct_count;
   a.fa = FillArgs{};
   const int nfind = (with_find && make_fill_args(c, P.map_old, target, a.fa)) ? P.map_np[P.map_old] * P.nseg : 0;
   *blocks = a.nsegs + nfind;
@@ -1231,35 +1519,29 @@ ExpandArgs expand_args(Ctx& c, int target, bool with_find, int* blocks) {
 }
 
 // fused: the lubrication forces ride in the same grid
+// fused (mfc_step): lubrication blocks, and the map buffer the coming links
+// rebuild (map_cur ^ 1, built two steps ago) cleared in the same grid and
+// marked empty
 BounceArgs bounce_args(Ctx& c, bool fused, int* blocks) {
   Particles& P = c.pt;
   BounceArgs a;
   a.A = c.pop[c.cur]; a.g = c.g; a.G = pgeom(c); a.mask = P.mask[P.map_cur]; a.nseg = P.nseg;
   a.off = P.lk_off; a.cap = P.lk_cap; a.lk_node = P.lk_node; a.lk_dir = P.lk_dir;
-  a.lk_alpha = P.lk_alpha; a.lk_out = P.lk_out; a.ypglb = P.ypglb; a.wp = P.wp; a.omgp = P.omgp;
+  a.lk_alpha = P.lk_alpha; a.lk_out = P.lk_out; a.ct_list = P.ct_list; a.ct_count = P.ct_count;
+  a.ypglb = P.ypglb; a.wp = P.wp; a.omgp = P.omgp;
   a.ww1 = c.p.ww1; a.ww2 = c.p.ww2; a.seg_ft = P.seg_ft; a.status = c.d_status;
   a.nsegs = P.np * P.nseg; a.np = P.np; a.lub = lub_args(c);
-  *blocks = a.nsegs + (fused ? P.np : 0);
-  return a;
-}
-
-// fused: the map buffer the coming links rebuild (map_cur ^ 1, built two
-// steps ago) is cleared in the same grid, and marked empty
-ScatterArgs scatter_args(Ctx& c, bool fused, int* blocks) {
-  Particles& P = c.pt;
+  a.nlub = fused ? lub_blocks(c) : 0;
   const int t = P.map_cur ^ 1;
   const int nclr = (fused && P.map_np[t] > 0) ? P.map_np[t] * P.nseg : 0;
-  ScatterArgs a;
-  a.A = c.pop[c.cur]; a.g = c.g; a.nseg = P.nseg; a.off = P.lk_off; a.cap = P.lk_cap;
-  a.lk_node = P.lk_node; a.lk_dir = P.lk_dir; a.lk_out = P.lk_out; a.seg_ft = P.seg_ft;
-  a.fHIp = P.fHIp; a.torqp = P.torqp; a.nsegs = P.np * P.nseg; a.seg_cap = P.seg_cap;
+  a.seg_cap = P.seg_cap;
   a.clr_list = P.sol_list[t]; a.clr_cnt = P.sol_cnt[t]; a.clr_isn = P.isn[t]; a.clr_mask = P.mask[t];
   if (nclr) P.map_np[t] = 0;
-  *blocks = a.nsegs + nclr;
+  *blocks = a.nsegs + a.nlub + nclr;
   return a;
 }
 
-MoveKArgs move_args(Ctx& c, int first, int wrap, int* blocks) {
+MoveKArgs move_args(Ctx& c, int first, int wrap, bool from_seg, int* blocks) {
   Particles& P = c.pt;
   MoveKArgs a;
   MoveArgs& m = a.m;
@@ -1272,6 +1554,7 @@ MoveKArgs move_args(Ctx& c, int first, int wrap, int* blocks) {
   m.bfx = c.p.body_force[0]; m.bfy = c.p.body_force[1]; m.bfz = c.p.body_force[2];
   m.pfx = c.p.particle_force[0]; m.pfy = c.p.particle_force[1]; m.pfz = c.p.particle_force[2];
   a.np = P.np; a.first = first; a.wrap = wrap; a.G = pgeom(c); a.status = c.d_status;
+  a.seg_ft = from_seg ? P.seg_ft : nullptr; a.nseg = P.nseg;
   *blocks = (P.np + 255) / 256;
   return a;
 }
@@ -1292,6 +1575,10 @@ hipError_t launch_links(Ctx& c, int target, int overlap_check, int istep, bool w
   if (e != hipSuccess) return e;
   if (P.np <= 0) return hipGetLastError();
   int nexp = 0;
+  if (overlap_check) {
+    e = ensure_cells(c);
+    if (e != hipSuccess) return e;
+  }
   const ExpandArgs ea = expand_args(c, target, with_find, &nexp);
   hipLaunchKernelGGL(k_links_scan, dim3(nsegs), dim3(PB), sizeof(uint32_t) * (size_t)P.seg_cap,
                      c.stream, scan_args(c, target, overlap_check, istep));
@@ -1301,31 +1588,43 @@ hipError_t launch_links(Ctx& c, int target, int overlap_check, int istep, bool w
   return hipGetLastError();
 }
 
+constexpr int kContactBlocks = 16;   // grid of the contact pre-pass (contacts are rare)
+
 hipError_t launch_bounce_back(Ctx& c, bool fused) {
-  int nb = 0, ns = 0;
-  const BounceArgs ba = bounce_args(c, fused, &nb);
-  const ScatterArgs sa = scatter_args(c, fused, &ns);
-  if (c.prec == MFC_PREC_FP64) {
-    hipLaunchKernelGGL(k_bounce_back<double>, dim3(nb), dim3(PB), 0, c.stream, ba);
-    hipLaunchKernelGGL(k_bb_scatter<double>, dim3(ns), dim3(PB), 0, c.stream, sa);
-  } else {
-    hipLaunchKernelGGL(k_bounce_back<float>, dim3(nb), dim3(PB), 0, c.stream, ba);
-    hipLaunchKernelGGL(k_bb_scatter<float>, dim3(ns), dim3(PB), 0, c.stream, sa);
+  Particles& P = c.pt;
+  if (fused) {
+    const hipError_t e = ensure_cells(c);
+    if (e != hipSuccess) return e;
   }
+  int nb = 0;
+  const BounceArgs ba = bounce_args(c, fused, &nb);
+  if (c.prec == MFC_PREC_FP64) {
+    hipLaunchKernelGGL(k_bb_contact<double>, dim3(kContactBlocks), dim3(PB), 0, c.stream, ba, kContactBlocks);
+    hipLaunchKernelGGL(k_bounce_back<double>, dim3(nb), dim3(PB), 0, c.stream, ba);
+  } else {
+    hipLaunchKernelGGL(k_bb_contact<float>, dim3(kContactBlocks), dim3(PB), 0, c.stream, ba, kContactBlocks);
+    hipLaunchKernelGGL(k_bounce_back<float>, dim3(nb), dim3(PB), 0, c.stream, ba);
+  }
+  if (!fused)
+    hipLaunchKernelGGL(k_seg_reduce, dim3((P.np + 255) / 256), dim3(256), 0, c.stream, P.seg_ft, P.nseg,
+                       P.np, P.fHIp, P.torqp);
   return hipGetLastError();
 }
 
 hipError_t launch_lubforce(Ctx& c) {
   Particles& P = c.pt;
-  hipLaunchKernelGGL(k_lubforce, dim3(P.np), dim3(PB), 0, c.stream, P.ypglb, P.np, pgeom(c),
-                     lub_args(c));
+  const hipError_t e = ensure_cells(c);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_lubforce, dim3(lub_blocks(c)), dim3(PB), 0, c.stream, P.ypglb, P.np, pgeom(c),
+                     lub_args(c), c.d_status);
   return hipGetLastError();
 }
 
-hipError_t launch_move(Ctx& c, int first, int wrap) {
+hipError_t launch_move(Ctx& c, int first, int wrap, bool from_seg) {
   int nb = 0;
-  const MoveKArgs a = move_args(c, first, wrap, &nb);
+  const MoveKArgs a = move_args(c, first, wrap, from_seg, &nb);
   hipLaunchKernelGGL(k_move, dim3(nb), dim3(256), 0, c.stream, a);
+  if (wrap) c.pt.cells_ok = false;   // the wrapped positions are the new ypglb
   return hipGetLastError();
 }
 
@@ -1333,8 +1632,11 @@ hipError_t launch_redistribute(Ctx& c, int istep) {
   Particles& P = c.pt;
   hipLaunchKernelGGL(k_wrap, dim3((P.np + 255) / 256), dim3(256), 0, c.stream, P.yp, P.ypglb,
                      P.ypglbp, P.np, pgeom(c), c.d_status);
+  P.cells_ok = false;
+  const hipError_t e = ensure_cells(c);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_overlap, dim3(P.np), dim3(PB), 0, c.stream, P.ypglb, P.np, pgeom(c), istep,
-                     c.d_status);
+                     cell_args(c), c.d_status);
   return hipGetLastError();
 }
 
@@ -1392,7 +1694,7 @@ namespace mfc {
 size_t stage_arg_bytes(Stage st) {
   switch (st) {
     case SG_BOUNCE: return sizeof(BounceArgs);
-    case SG_SCATTER: return sizeof(ScatterArgs);
+    case SG_CONTACT: return sizeof(BounceArgs);
     case SG_MOVE: return sizeof(MoveKArgs);
     case SG_SCAN: return sizeof(ScanArgs);
     case SG_COUNT: return sizeof(ScanCountArgs);
@@ -1413,11 +1715,11 @@ hipError_t launch_batched(Stage st, int prec, int walls, const void* t, const in
       else
         hipLaunchKernelGGL(kb_bounce_back<float>, dim3(blocks), dim3(PB), 0, s, (const BounceArgs*)t, off, nc);
       break;
-    case SG_SCATTER:
+    case SG_CONTACT:
       if (f64)
-        hipLaunchKernelGGL(kb_bb_scatter<double>, dim3(blocks), dim3(PB), 0, s, (const ScatterArgs*)t, off, nc);
+        hipLaunchKernelGGL(kb_bb_contact<double>, dim3(blocks), dim3(PB), 0, s, (const BounceArgs*)t, off, nc);
       else
-        hipLaunchKernelGGL(kb_bb_scatter<float>, dim3(blocks), dim3(PB), 0, s, (const ScatterArgs*)t, off, nc);
+        hipLaunchKernelGGL(kb_bb_contact<float>, dim3(blocks), dim3(PB), 0, s, (const BounceArgs*)t, off, nc);
       break;
     case SG_MOVE:
       hipLaunchKernelGGL(kb_move, dim3(blocks), dim3(256), 0, s, (const MoveKArgs*)t, off, nc);

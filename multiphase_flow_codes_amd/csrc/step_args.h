@@ -49,7 +49,17 @@ struct FillArgs {
   int64_t cap;
 };
 
+// Uniform cell grid of the particle pair searches (lubrication, overlap
+// check): cells at least as wide as the longest pair range, at most
+// kCellCap particles each.  cnt == nullptr: all pairs are searched.
+struct CellArgs {
+  int32_t* cnt;
+  int32_t* list;   // kCellCap slots per cell
+  int ncx, ncy, ncz;
+};
+
 struct LubArgs {
+  CellArgs cells;
   double mingap, fpc, stf0, stf1;
   double mingap_w, stf0_w, stf1_w;   // sphere-wall lubrication (walls_y)
   double* flubp;
@@ -80,6 +90,7 @@ struct ScanArgs {
   int ovl;
   int istep;
   int32_t* status;
+  CellArgs cells;
 };
 
 struct ScanCountArgs {
@@ -90,6 +101,7 @@ struct ScanCountArgs {
   int64_t cap;
   int32_t* status;
   int32_t* fill_count;
+  int32_t* ct_count;
 };
 
 struct ExpandArgs {
@@ -109,10 +121,15 @@ struct ExpandArgs {
   int32_t* status;
   int nsegs;
   FillArgs fa;
+  const uint32_t* mask;   // the map being built: links whose node is solid are contact links
+  uint64_t* ct_list;
+  int32_t* ct_count;
 };
 
+// Arguments of the bounce-back grid (link segments, then nlub lubrication
+// blocks, then the map-clearing blocks) and of its contact pre-pass.
 struct BounceArgs {
-  const void* A;
+  void* A;
   Grid g;
   PGeom G;
   const uint32_t* mask;
@@ -122,7 +139,9 @@ struct BounceArgs {
   const uint32_t* lk_node;
   const uint8_t* lk_dir;
   const double* lk_alpha;
-  double* lk_out;
+  double2* lk_out;              // (f9, ff1) of the contact links
+  const uint64_t* ct_list;      // contact links: (particle << 40) | link
+  const int32_t* ct_count;
   const double* ypglb;
   const double* wp;
   const double* omgp;
@@ -132,23 +151,9 @@ struct BounceArgs {
   int32_t* status;
   int nsegs;
   int np;
+  int nlub;                     // lubrication blocks (np in the fused step, else 0)
   LubArgs lub;
-};
-
-struct ScatterArgs {
-  void* A;
-  Grid g;
-  int nseg;
-  const int64_t* off;
-  int64_t cap;
-  const uint32_t* lk_node;
-  const uint8_t* lk_dir;
-  const double* lk_out;
-  const double* seg_ft;
-  double* fHIp;
-  double* torqp;
-  int nsegs;
-  int seg_cap;
+  int seg_cap;                  // map clearing (clr_* of map buffer map_cur ^ 1)
   const uint32_t* clr_list;
   const int32_t* clr_cnt;
   int32_t* clr_isn;
@@ -162,6 +167,8 @@ struct MoveKArgs {
   int wrap;
   PGeom G;
   int32_t* status;
+  const double* seg_ft;   // non-null: fHIp / torqp are first summed from the segments
+  int nseg;
 };
 
 struct FillApplyArgs {
@@ -177,13 +184,12 @@ ScanArgs scan_args(Ctx& c, int target, int ovl, int istep);
 ScanCountArgs count_args(Ctx& c, int target);
 ExpandArgs expand_args(Ctx& c, int target, bool with_find, int* blocks);
 BounceArgs bounce_args(Ctx& c, bool fused, int* blocks);
-ScatterArgs scatter_args(Ctx& c, bool fused, int* blocks);
-MoveKArgs move_args(Ctx& c, int first, int wrap, int* blocks);
+MoveKArgs move_args(Ctx& c, int first, int wrap, bool from_seg, int* blocks);
 bool fill_apply_args(Ctx& c, int nblk, FillApplyArgs& a);
 
 // ---- launches over several boxes: t = device table of the stage's argument
 // blocks (one per box), off = device block offsets (nc + 1), `blocks` = off[nc]
-enum Stage { SG_BOUNCE, SG_SCATTER, SG_MOVE, SG_SCAN, SG_COUNT, SG_EXPAND, SG_FILL };
+enum Stage { SG_CONTACT, SG_BOUNCE, SG_MOVE, SG_SCAN, SG_COUNT, SG_EXPAND, SG_FILL };
 hipError_t launch_batched(Stage st, int prec, int walls, const void* t, const int* off, int nc,
                           int blocks, size_t lds, hipStream_t s);
 hipError_t launch_batched_collide(int prec, int walls, int bs, const void* t, const int* off, int nc,
