@@ -225,6 +225,38 @@ def calibrate_copy(mib):
     log(f"copy calibration: 5 copies of {mib} MiB done")
 
 
+def hbm_ceilings(gib=4, reps=5):
+    """Device bandwidth ceilings with simple kernels over `gib` GiB buffers:
+    runtime copy, out-of-place elementwise (read one buffer, write another),
+    in-place elementwise (read and write the same buffer), write-only fill."""
+    import torch
+    n = gib * (1 << 30) // 4
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    nbytes = gib * (1 << 30)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    out = {}
+    for name, fn, moved in (("copy", lambda: b.copy_(a), 2),
+                            ("elementwise_out_of_place", lambda: torch.add(a, 1.0, out=b), 2),
+                            ("elementwise_in_place", lambda: a.mul_(1.0000001), 2),
+                            ("fill", lambda: b.fill_(0.5), 1)):
+        ms = timed(fn)
+        out[name] = round(moved * nbytes / ms / 1e6, 1)
+        log(f"ceiling {name}: {ms:.3f} ms for {gib} GiB -> {out[name]:.0f} GB/s")
+    print(json.dumps({"hbm_ceilings_GBps": out, "buffer_GiB": gib}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -257,9 +289,14 @@ def main():
     ap.add_argument("--config3-no-fp64", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized buffer 5 times (HBM counter calibration)")
+    ap.add_argument("--ceilings", action="store_true",
+                    help="only measure the device's bandwidth ceilings with simple kernels")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.ceilings:
+        hbm_ceilings()
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a "

@@ -12,6 +12,7 @@
 // macrovar), valid while the field is unchanged since that macrovar.
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -312,6 +313,10 @@ int mfc_create(const mfc_params* p, mfc_ctx** out) {
                        p->body_force[2]};
   c.prec = p->precision;
   c.tsize = c.prec == MFC_PREC_FP64 ? 8 : 4;
+  {
+    const char* e = getenv("MFC_XCD_REMAP");
+    c.xcd_remap = (e && e[0] == '0') ? 0 : 1;
+  }
   c.iseedf = p->iseedf;
   int rc = MFC_OK;
   auto bail = [&](int code) {
@@ -378,7 +383,7 @@ int mfc_destroy(mfc_ctx* ctx) {
   for (double* a : pd) F(a);
   F(P.lk_out); F(P.ct_list); F(P.ct_count);
   F(P.isn[0]); F(P.isn[1]); F(P.mask[0]); F(P.mask[1]);
-  F(P.lk_node); F(P.lk_dir); F(P.lk_cnt); F(P.lk_off);
+  F(P.lk_node); F(P.lk_dir); F(P.lk_cnt); F(P.lk_off); F(P.lk_part);
   F(P.sol_list[0]); F(P.sol_list[1]); F(P.sol_cnt[0]); F(P.sol_cnt[1]);
   F(P.sh_node); F(P.sh_hits); F(P.sh_cnt);
   F(P.fill_list); F(P.fill_count); F(P.cell_cnt); F(P.cell_list);
@@ -693,6 +698,8 @@ int mfc_prt_init(mfc_ctx* ctx, int32_t npart) {
     }
     HIPCHK(hipMalloc(&P.lk_cnt, sizeof(int32_t) * nsegs));
     HIPCHK(hipMalloc(&P.lk_off, sizeof(int64_t) * (nsegs + 1)));
+    if (P.lk_part) HIPCHK(hipFree(P.lk_part));
+    HIPCHK(hipMalloc(&P.lk_part, sizeof(int64_t) * 2 * (nsegs / kScanChunk + 1)));
     for (int b = 0; b < 2; ++b) {
       HIPCHK(hipMalloc(&P.sol_list[b], sizeof(uint32_t) * nodes));
       HIPCHK(hipMalloc(&P.sol_cnt[b], sizeof(int32_t) * nsegs));

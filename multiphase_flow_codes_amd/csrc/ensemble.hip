@@ -131,9 +131,9 @@ int ens_one_step(mfc_ens* e) {
       c.f_version++;
     }
     if (on) {
-      int nb = 0;
-      const BounceArgs ba = bounce_args(c, true, &nb);
-      st[0].add(&ba, kContactBlocksPerBox);
+      int nb = 0, npre = 0;
+      const BounceArgs ba = bounce_args(c, true, kContactBlocksPerBox, &nb, &npre);
+      st[0].add(&ba, npre);
       st[1].add(&ba, nb);
       c.f_version++;
     }

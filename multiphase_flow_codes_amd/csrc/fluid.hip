@@ -177,11 +177,26 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
   }
 }
 
+// Workgroups are dealt to the XCDs round-robin in launch order, and each XCD
+// has its own L2.  Tile t (x-chunk fastest, then y, then z) of the box is
+// given to launch index L so that each XCD takes a contiguous range of tiles:
+// the x-chunks of a row, whose loads and stores share cache lines at the
+// chunk edges, and the neighbouring rows run on one XCD.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t L, uint32_t total) {
+  const uint32_t per = total / kXcds;
+  if (L >= per * kXcds) return L;   // the last total % 8 tiles keep their index
+  return (L % kXcds) * per + L / kXcds;
+}
+
 template <typename T, typename R, bool ARR, bool WALLS, int LAY>
-__global__ void __launch_bounds__(256) k_collide(CollideArgs a) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_collide(CollideArgs a, int nbx, int remap) {
+  const uint32_t t = remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+  const uint32_t bx = t % (uint32_t)nbx, row = t / (uint32_t)nbx;
+  const int x = (int)(bx * blockDim.x + threadIdx.x);
   if (x >= a.g.nx) return;
-  collide_node<T, R, ARR, WALLS, LAY>(a, x, blockIdx.y, blockIdx.z);
+  const uint32_t y = row % (uint32_t)a.g.ny, z = row / (uint32_t)a.g.ny;
+  collide_node<T, R, ARR, WALLS, LAY>(a, x, (int)y, (int)z);
 }
 
 // Several boxes in one grid (ensemble engine): block b belongs to box c with
@@ -449,12 +464,14 @@ hipError_t launch_force_table(Ctx& c, int slot, const ModeSet& m) {
   return hipGetLastError();
 }
 
-// x-row blocks: 64..256 lanes covering a row, one block per (x-chunk, y, z)
-static void row_launch(const Grid& g, dim3& grid, dim3& blk) {
+// x-row blocks: 64..256 lanes covering a row, one block per (x-chunk, y, z),
+// a one-dimensional grid of nbx * ny * nz tiles
+static void row_launch(const Grid& g, dim3& grid, dim3& blk, int& nbx) {
   int bx = ((g.nx + 63) / 64) * 64;
   if (bx > 256) bx = 256;
   blk = dim3(bx);
-  grid = dim3((g.nx + bx - 1) / bx, g.ny, g.nz);
+  nbx = (g.nx + bx - 1) / bx;
+  grid = dim3((unsigned)((uint64_t)nbx * g.ny * g.nz));
 }
 
 // Arguments of one box's collide launch (also used by the ensemble engine).
@@ -474,17 +491,19 @@ CollideArgs collide_args(Ctx& c) {
 template <typename T, typename R, int LAY>
 static void collide_dispatch_lay(Ctx& c, bool arrays, const CollideArgs& a) {
   dim3 grid, blk;
-  row_launch(c.g, grid, blk);
+  int nbx;
+  row_launch(c.g, grid, blk, nbx);
+  const int rm = c.xcd_remap;
   if (arrays) {
     if (c.g.walls_y)
-      hipLaunchKernelGGL((k_collide<T, R, true, true, LAY>), grid, blk, 0, c.stream, a);
+      hipLaunchKernelGGL((k_collide<T, R, true, true, LAY>), grid, blk, 0, c.stream, a, nbx, rm);
     else
-      hipLaunchKernelGGL((k_collide<T, R, true, false, LAY>), grid, blk, 0, c.stream, a);
+      hipLaunchKernelGGL((k_collide<T, R, true, false, LAY>), grid, blk, 0, c.stream, a, nbx, rm);
   } else {
     if (c.g.walls_y)
-      hipLaunchKernelGGL((k_collide<T, R, false, true, LAY>), grid, blk, 0, c.stream, a);
+      hipLaunchKernelGGL((k_collide<T, R, false, true, LAY>), grid, blk, 0, c.stream, a, nbx, rm);
     else
-      hipLaunchKernelGGL((k_collide<T, R, false, false, LAY>), grid, blk, 0, c.stream, a);
+      hipLaunchKernelGGL((k_collide<T, R, false, false, LAY>), grid, blk, 0, c.stream, a, nbx, rm);
   }
 }
 

@@ -37,6 +37,9 @@ constexpr int kForceSlots = 4;
 // (between 1 and 4/3 times 2r + mingap wide) holds at most kCellCap centres
 // (non-overlapping spheres never come close to it; more is reported)
 constexpr int kCellMinParticles = 512;
+// segment-count scans of more than kScanChunk segments run over several
+// workgroups, kScanChunk segments each
+constexpr int kScanChunk = 4096;
 constexpr int kCellCap = 16;
 // The live forcing modes of one FORCING call, grouped by kx (n[0] modes with
 // kx = 0 first, then kx = 1, then kx = 2), each in (kz, ky) order, with
@@ -95,6 +98,7 @@ struct Particles {
   uint64_t* ct_list = nullptr; // contact links of the last build: (particle << 40) | link
   int32_t* ct_count = nullptr;
   int32_t* lk_cnt = nullptr;   // links per segment
+  int64_t* lk_part = nullptr;  // per-chunk sums of the segment counts (multi-block scan)
   int64_t* lk_off = nullptr;   // link offset per segment (np * nseg + 1)
   double* seg_ft = nullptr;    // force and torque per segment (6 each)
   // cell grid of the pair searches (on for many particles): per-cell counts
@@ -149,6 +153,7 @@ struct Ctx {
   double2* diag_Z = nullptr;        // complex FP64 field
   double2* diag_tw[3] = {nullptr, nullptr, nullptr};   // exp(-2 pi i j / n) per axis
   void* ens = nullptr;              // the ensemble this box is a member of (its stream is on loan)
+  int xcd_remap = 1;                // collide tiles grouped per XCD (MFC_XCD_REMAP=0 turns it off)
 };
 
 // ---- launchers (fluid.hip) ----

@@ -809,14 +809,25 @@ __device__ __forceinline__ void link_bounce(const T* A, const Grid& g, const PGe
   }
 }
 
-// Contact links (node x solid for another sphere, spheres almost touching)
-// read post-collision values at a solid node, which the bounce-back grid
-// overwrites: their values are taken first, from the untouched field, into
-// lk_out (f9, ff1).  Every other link reads fluid nodes only and writes only
-// solid ones, so the bounce-back grid can scatter as it goes; all links see
-// the one post-collision snapshot.  Grid-stride over the contact list.
+// Pre-pass of the bounce-back.  Contact links (node x solid for another
+// sphere, spheres almost touching) read post-collision values at a solid
+// node, which the bounce-back grid overwrites: their values are taken first,
+// from the untouched field, into lk_out (f9, ff1).  Every other link reads
+// fluid nodes only and writes only solid ones, so the bounce-back grid can
+// scatter as it goes; all links see the one post-collision snapshot.  The
+// first nct blocks stride over the contact list; the blocks after them
+// compute beads_lubforce (which the bounce-back does not read).
 template <typename T>
-__device__ __forceinline__ void bb_contact_body(int b_, int nblk, const BounceArgs& a) {
+__device__ __forceinline__ void bb_contact_body(int b_, const BounceArgs& a) {
+  if (b_ >= a.nct) {
+    const int r = b_ - a.nct;
+    if (a.lub.cells.cnt)
+      lub_cells(r * PB + (int)threadIdx.x, a.ypglb, a.np, a.G, a.lub, a.status);
+    else
+      lub_block(r, a.ypglb, a.np, a.G, a.lub);
+    return;
+  }
+  const int nblk = a.nct;
   const T* A = (const T*)a.A;
   const int64_t total = min((int64_t)*a.ct_count, a.cap);
   for (int64_t e = (int64_t)b_ * PB + threadIdx.x; e < total; e += (int64_t)nblk * PB) {
@@ -839,9 +850,8 @@ __device__ __forceinline__ void bb_contact_body(int b_, int nblk, const BounceAr
 // the bounce-back value of every link, written at once into the opposite
 // population of the solid neighbour x + c (partlib.f90:1605-1621), and the
 // segment's momentum-exchange force and torque (FP64, fixed reduction order)
-// into seg_ft.  Blocks past the link segments compute beads_lubforce of one
-// particle, then (fused step) reset one segment of the map buffer the coming
-// beads_links rebuilds (built two steps ago).
+// into seg_ft.  Blocks past the link segments (fused step) reset one segment
+// of the map buffer the coming beads_links rebuilds (built two steps ago).
 template <typename T>
 __device__ __forceinline__ void bounce_back_body(int b_, const BounceArgs& a) {
   __shared__ double red[6][PB / 64];
@@ -849,14 +859,7 @@ __device__ __forceinline__ void bounce_back_body(int b_, const BounceArgs& a) {
   const Grid& g = a.g;
   const int b = b_;
   if (b >= a.nsegs) {
-    const int r = b - a.nsegs;
-    if (r < a.nlub) {
-      if (a.lub.cells.cnt)
-        lub_cells(r * PB + (int)threadIdx.x, a.ypglb, a.np, a.G, a.lub, a.status);
-      else
-        lub_block(r, a.ypglb, a.np, a.G, a.lub);
-    } else
-      clear_segment(r - a.nlub, a.nseg, a.seg_cap, a.clr_list, a.clr_cnt, a.clr_isn, a.clr_mask);
+    clear_segment(b - a.nsegs, a.nseg, a.seg_cap, a.clr_list, a.clr_cnt, a.clr_isn, a.clr_mask);
     return;
   }
   const int p = b / a.nseg;
@@ -1260,6 +1263,89 @@ __global__ void __launch_bounds__(PB)
   links_scan_body((int)blockIdx.x - off[c], a.ypglb, a.np, a.nseg, a.seg_cap, a.G, a.isn, a.mask, a.sol_list, a.sol_cnt, a.sh_node, a.sh_hits, a.sh_cnt, a.lk_cnt, a.ovl, a.istep, a.status, a.cells);
 }
 
+// The same scan over many segments, in two grids of kScanChunk-segment
+// chunks: each chunk's sums, then each chunk scans itself after adding the
+// chunks before it.  Integer sums: the result equals the one-block scan.
+constexpr int kScanPer = kScanChunk / PB;
+__global__ void __launch_bounds__(PB) k_count_partials(ScanCountArgs a) {
+  __shared__ int sh[PB / 64];
+  const int b = blockIdx.x;
+  int s = 0, ss = 0;
+  for (int k = 0; k < kScanPer; ++k) {
+    const int i = b * kScanChunk + k * PB + (int)threadIdx.x;
+    if (i < a.nsegs) {
+      s += a.cnt[i];
+      ss += a.scnt[i];
+    }
+  }
+  int t1, t2;
+  block_exclusive_scan(s, sh, t1);
+  block_exclusive_scan(ss, sh, t2);
+  if (threadIdx.x == 0) {
+    a.part[2 * b] = t1;
+    a.part[2 * b + 1] = t2;
+  }
+}
+
+__global__ void __launch_bounds__(PB) k_count_offsets(ScanCountArgs a) {
+  __shared__ int sh[PB / 64];
+  __shared__ int64_t red[2][PB];
+  const int b = blockIdx.x;
+  if (b == 0 && threadIdx.x == 0) {
+    *a.fill_count = 0;   // the next beads_filling starts an empty list
+    *a.ct_count = 0;     // the expansion lists this build's contact links
+  }
+  // chunks before this one (and all chunks, for the totals)
+  const int nb = gridDim.x;
+  int64_t before = 0, tot = 0, stot = 0;
+  for (int k = threadIdx.x; k < nb; k += PB) {
+    const int64_t v = a.part[2 * k];
+    tot += v;
+    stot += a.part[2 * k + 1];
+    if (k < b) before += v;
+  }
+  red[0][threadIdx.x] = before;
+  red[1][threadIdx.x] = tot;
+  __syncthreads();
+  for (int o = PB / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + o];
+      red[1][threadIdx.x] += red[1][threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  const int64_t base = red[0][0], total = red[1][0];
+  __syncthreads();
+  // this chunk: thread t owns segments [t * kScanPer, (t + 1) * kScanPer)
+  const int i0 = b * kScanChunk + (int)threadIdx.x * kScanPer;
+  int s = 0;
+  for (int k = 0; k < kScanPer; ++k)
+    if (i0 + k < a.nsegs) s += a.cnt[i0 + k];
+  int ct;
+  int64_t acc = base + block_exclusive_scan(s, sh, ct);
+  for (int k = 0; k < kScanPer; ++k) {
+    if (i0 + k < a.nsegs) {
+      a.off[i0 + k] = acc;
+      acc += a.cnt[i0 + k];
+    }
+  }
+  if (b == nb - 1) {
+    // every chunk's solid count, by the same reduction
+    red[1][threadIdx.x] = stot;
+    __syncthreads();
+    for (int o = PB / 2; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) red[1][threadIdx.x] += red[1][threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      a.off[a.nsegs] = total;
+      a.status[ST_NLINKS] = (int32_t)min(total, (int64_t)0x7fffffff);
+      a.status[ST_NSOLID] = (int32_t)min(red[1][0], (int64_t)0x7fffffff);
+      if (total > a.cap) atomicAdd(&a.status[ST_LINK_OVERFLOW], 1);
+    }
+  }
+}
+
 __global__ void __launch_bounds__(1024) k_scan_counts(ScanCountArgs a) {
   scan_counts_body((int)blockIdx.x, a.cnt, a.scnt, a.nsegs, a.off, a.cap, a.status, a.fill_count, a.ct_count);
 }
@@ -1281,15 +1367,15 @@ __global__ void __launch_bounds__(PB)
 }
 
 template <typename T>
-__global__ void __launch_bounds__(PB) k_bb_contact(BounceArgs a, int nblk) {
-  bb_contact_body<T>((int)blockIdx.x, nblk, a);
+__global__ void __launch_bounds__(PB) k_bb_contact(BounceArgs a) {
+  bb_contact_body<T>((int)blockIdx.x, a);
 }
 template <typename T>
 __global__ void __launch_bounds__(PB)
     kb_bb_contact(const BounceArgs* __restrict__ t, const int* __restrict__ off, int nc) {
   const int c = case_of(off, nc, (int)blockIdx.x);
   const BounceArgs a = t[c];
-  bb_contact_body<T>((int)blockIdx.x - off[c], off[c + 1] - off[c], a);
+  bb_contact_body<T>((int)blockIdx.x - off[c], a);
 }
 
 template <typename T>
@@ -1499,6 +1585,7 @@ ScanCountArgs count_args(Ctx& c, int target) {
   ScanCountArgs a;
   a.cnt = P.lk_cnt; a.scnt = P.sol_cnt[target]; a.nsegs = P.np * P.nseg; a.off = P.lk_off;
   a.cap = P.lk_cap; a.status = c.d_status; a.fill_count = P.fill_count; a.ct_count = P.ct_count;
+  a.part = P.lk_part;
   return a;
 }
 
@@ -1522,7 +1609,7 @@ ExpandArgs expand_args(Ctx& c, int target, bool with_find, int* blocks) {
 // fused (mfc_step): lubrication blocks, and the map buffer the coming links
 // rebuild (map_cur ^ 1, built two steps ago) cleared in the same grid and
 // marked empty
-BounceArgs bounce_args(Ctx& c, bool fused, int* blocks) {
+BounceArgs bounce_args(Ctx& c, bool fused, int nct, int* bb_blocks, int* pre_blocks) {
   Particles& P = c.pt;
   BounceArgs a;
   a.A = c.pop[c.cur]; a.g = c.g; a.G = pgeom(c); a.mask = P.mask[P.map_cur]; a.nseg = P.nseg;
@@ -1531,13 +1618,15 @@ BounceArgs bounce_args(Ctx& c, bool fused, int* blocks) {
   a.ypglb = P.ypglb; a.wp = P.wp; a.omgp = P.omgp;
   a.ww1 = c.p.ww1; a.ww2 = c.p.ww2; a.seg_ft = P.seg_ft; a.status = c.d_status;
   a.nsegs = P.np * P.nseg; a.np = P.np; a.lub = lub_args(c);
+  a.nct = nct;
   a.nlub = fused ? lub_blocks(c) : 0;
   const int t = P.map_cur ^ 1;
   const int nclr = (fused && P.map_np[t] > 0) ? P.map_np[t] * P.nseg : 0;
   a.seg_cap = P.seg_cap;
   a.clr_list = P.sol_list[t]; a.clr_cnt = P.sol_cnt[t]; a.clr_isn = P.isn[t]; a.clr_mask = P.mask[t];
   if (nclr) P.map_np[t] = 0;
-  *blocks = a.nsegs + a.nlub + nclr;
+  *bb_blocks = a.nsegs + nclr;
+  *pre_blocks = a.nct + a.nlub;
   return a;
 }
 
@@ -1582,7 +1671,14 @@ hipError_t launch_links(Ctx& c, int target, int overlap_check, int istep, bool w
   const ExpandArgs ea = expand_args(c, target, with_find, &nexp);
   hipLaunchKernelGGL(k_links_scan, dim3(nsegs), dim3(PB), sizeof(uint32_t) * (size_t)P.seg_cap,
                      c.stream, scan_args(c, target, overlap_check, istep));
-  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, c.stream, count_args(c, target));
+  if (nsegs > kScanChunk) {
+    const int nb = (nsegs + kScanChunk - 1) / kScanChunk;
+    const ScanCountArgs ca = count_args(c, target);
+    hipLaunchKernelGGL(k_count_partials, dim3(nb), dim3(PB), 0, c.stream, ca);
+    hipLaunchKernelGGL(k_count_offsets, dim3(nb), dim3(PB), 0, c.stream, ca);
+  } else {
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, c.stream, count_args(c, target));
+  }
   hipLaunchKernelGGL(k_links_expand, dim3(nexp), dim3(PB), 0, c.stream, ea);
   P.map_np[target] = P.np;
   return hipGetLastError();
@@ -1596,13 +1692,13 @@ hipError_t launch_bounce_back(Ctx& c, bool fused) {
     const hipError_t e = ensure_cells(c);
     if (e != hipSuccess) return e;
   }
-  int nb = 0;
-  const BounceArgs ba = bounce_args(c, fused, &nb);
+  int nb = 0, npre = 0;
+  const BounceArgs ba = bounce_args(c, fused, kContactBlocks, &nb, &npre);
   if (c.prec == MFC_PREC_FP64) {
-    hipLaunchKernelGGL(k_bb_contact<double>, dim3(kContactBlocks), dim3(PB), 0, c.stream, ba, kContactBlocks);
+    hipLaunchKernelGGL(k_bb_contact<double>, dim3(npre), dim3(PB), 0, c.stream, ba);
     hipLaunchKernelGGL(k_bounce_back<double>, dim3(nb), dim3(PB), 0, c.stream, ba);
   } else {
-    hipLaunchKernelGGL(k_bb_contact<float>, dim3(kContactBlocks), dim3(PB), 0, c.stream, ba, kContactBlocks);
+    hipLaunchKernelGGL(k_bb_contact<float>, dim3(npre), dim3(PB), 0, c.stream, ba);
     hipLaunchKernelGGL(k_bounce_back<float>, dim3(nb), dim3(PB), 0, c.stream, ba);
   }
   if (!fused)

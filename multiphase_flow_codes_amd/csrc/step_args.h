@@ -102,6 +102,7 @@ struct ScanCountArgs {
   int32_t* status;
   int32_t* fill_count;
   int32_t* ct_count;
+  int64_t* part;          // per-chunk sums of the multi-block form (2 per chunk)
 };
 
 struct ExpandArgs {
@@ -126,8 +127,9 @@ struct ExpandArgs {
   int32_t* ct_count;
 };
 
-// Arguments of the bounce-back grid (link segments, then nlub lubrication
-// blocks, then the map-clearing blocks) and of its contact pre-pass.
+// Arguments of the bounce-back grid (link segments, then the map-clearing
+// blocks) and of its pre-pass grid (nct blocks for the contact links, then
+// nlub lubrication blocks).
 struct BounceArgs {
   void* A;
   Grid g;
@@ -151,7 +153,8 @@ struct BounceArgs {
   int32_t* status;
   int nsegs;
   int np;
-  int nlub;                     // lubrication blocks (np in the fused step, else 0)
+  int nct;                      // contact blocks of the pre-pass grid
+  int nlub;                     // lubrication blocks of the pre-pass grid (fused step, else 0)
   LubArgs lub;
   int seg_cap;                  // map clearing (clr_* of map buffer map_cur ^ 1)
   const uint32_t* clr_list;
@@ -183,7 +186,9 @@ CollideArgs collide_args(Ctx& c);
 ScanArgs scan_args(Ctx& c, int target, int ovl, int istep);
 ScanCountArgs count_args(Ctx& c, int target);
 ExpandArgs expand_args(Ctx& c, int target, bool with_find, int* blocks);
-BounceArgs bounce_args(Ctx& c, bool fused, int* blocks);
+// bb_blocks: the bounce-back grid; pre_blocks: the pre-pass grid (nct
+// contact blocks + lubrication)
+BounceArgs bounce_args(Ctx& c, bool fused, int nct, int* bb_blocks, int* pre_blocks);
 MoveKArgs move_args(Ctx& c, int first, int wrap, bool from_seg, int* blocks);
 bool fill_apply_args(Ctx& c, int nblk, FillApplyArgs& a);
 

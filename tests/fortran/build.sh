@@ -24,3 +24,6 @@ flags="-fdefault-real-8 -O2 -J $out"
 "$fc" $flags -c "$here/layout_check.f90" -o "$out/layout_check.o"
 "$fc" -o "$out/layout_check" "$out/layout_check.o" "$out/mfc_lbm_c.o"
 echo "fortran drop-in: built $out/dropin_driver, $out/layout_check"
+"$fc" $flags -c "$here/ckpt_io.f90" -o "$out/ckpt_io.o"
+"$fc" -o "$out/ckpt_io" "$out/ckpt_io.o"
+echo "fortran drop-in: built $out/ckpt_io"
