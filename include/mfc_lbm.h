@@ -203,6 +203,10 @@ int mfc_ens_sync(mfc_ens* e);
 /* Threads per collide block (64, 128 or 256), or chosen by timing each on the
  * current members (they advance 3 * steps steps); ms_per_step3 may be null. */
 int mfc_ens_set_block(mfc_ens* e, int32_t bs);
+/* on = 1: each step (its table upload and stage grids) is recorded once as a
+ * HIP graph per table buffer and replayed while membership and launch shapes
+ * stay the same (re-recorded when they change); on = 0: plain launches. */
+int mfc_ens_set_graph(mfc_ens* e, int32_t on);
 int mfc_ens_autotune(mfc_ens* e, int32_t steps, int32_t* chosen, double* ms_per_step3);
 void* mfc_ens_get_stream(mfc_ens* e);
 

@@ -55,6 +55,11 @@ struct mfc_ens {
   size_t h_cap[kRing] = {};
   hipEvent_t h_ev[kRing] = {};
   int ring = 0;
+  // graph mode: each ring slot's step (table copy + the stage grids) is
+  // captured once and replayed while the grid shapes stay the same
+  int use_graph = 0;
+  hipGraphExec_t gexec[kRing] = {};
+  std::vector<long long> gsig[kRing];
   std::string err;
 };
 
@@ -195,16 +200,59 @@ int ens_one_step(mfc_ens* e) {
     if (!tabs[k]->args.empty()) memcpy(h + pos[k][0], tabs[k]->args.data(), tabs[k]->args.size());
     memcpy(h + pos[k][1], tabs[k]->off.data(), sizeof(int) * tabs[k]->off.size());
   }
-  EHIP(hipMemcpyAsync(e->d_tab, h, total, hipMemcpyHostToDevice, e->stream));
-  EHIP(hipEventRecord(e->h_ev[r], e->stream));
   char* d = e->d_tab;
-  EHIP(launch_batched_collide(e->prec, e->walls, e->collide_bs, d + pos[0][0], (const int*)(d + pos[0][1]),
-                              coll.nc, coll.off.back(), e->stream));
-  for (int k = 0; k < 7; ++k) {
-    StageTab& t = st[k];
-    EHIP(launch_batched(order[k], e->prec, e->walls, d + pos[k + 1][0], (const int*)(d + pos[k + 1][1]),
-                        t.nc, t.off.back(), t.lds, e->stream));
+  auto enqueue = [&]() -> int {
+    EHIP(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, e->stream));
+    EHIP(launch_batched_collide(e->prec, e->walls, e->collide_bs, d + pos[0][0], (const int*)(d + pos[0][1]),
+                                coll.nc, coll.off.back(), e->stream));
+    for (int k = 0; k < 7; ++k) {
+      StageTab& t = st[k];
+      EHIP(launch_batched(order[k], e->prec, e->walls, d + pos[k + 1][0], (const int*)(d + pos[k + 1][1]),
+                          t.nc, t.off.back(), t.lds, e->stream));
+    }
+    return MFC_OK;
+  };
+  if (!e->use_graph) {
+    const int rc = enqueue();
+    if (rc) return rc;
+    EHIP(hipEventRecord(e->h_ev[r], e->stream));
+    return MFC_OK;
   }
+  // what the captured work depends on besides the table contents: buffers,
+  // sizes, launch shapes and the collide block size
+  std::vector<long long> sig = {(long long)(uintptr_t)d, (long long)(uintptr_t)h, (long long)total,
+                                e->collide_bs, e->prec, e->walls, coll.nc, coll.off.back()};
+  for (int k = 0; k < 7; ++k) {
+    sig.push_back(st[k].nc);
+    sig.push_back(st[k].off.back());
+    sig.push_back((long long)st[k].lds);
+  }
+  for (int k = 0; k < kStages; ++k) {
+    sig.push_back((long long)pos[k][0]);
+    sig.push_back((long long)pos[k][1]);
+  }
+  if (!e->gexec[r] || e->gsig[r] != sig) {
+    if (e->gexec[r]) EHIP(hipGraphExecDestroy(e->gexec[r]));
+    e->gexec[r] = nullptr;
+    hipGraph_t g = nullptr;
+    EHIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue();
+    const hipError_t end = hipStreamEndCapture(e->stream, &g);
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    if (end != hipSuccess) return efail(e, MFC_ERR_HIP, "hipStreamEndCapture", end);
+    const hipError_t inst = hipGraphInstantiate(&e->gexec[r], g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (inst != hipSuccess) {
+      e->gexec[r] = nullptr;
+      return efail(e, MFC_ERR_HIP, "hipGraphInstantiate", inst);
+    }
+    e->gsig[r] = sig;
+  }
+  EHIP(hipGraphLaunch(e->gexec[r], e->stream));
+  EHIP(hipEventRecord(e->h_ev[r], e->stream));
   return MFC_OK;
 }
 
@@ -245,6 +293,8 @@ int mfc_ens_destroy(mfc_ens* e) {
     if (e->h_tab[r]) hipHostFree(e->h_tab[r]);
     if (e->h_ev[r]) hipEventDestroy(e->h_ev[r]);
   }
+  for (int r = 0; r < kRing; ++r)
+    if (e->gexec[r]) hipGraphExecDestroy(e->gexec[r]);
   if (e->d_tab) hipFree(e->d_tab);
   if (e->stream) hipStreamDestroy(e->stream);
   delete e;
@@ -309,6 +359,13 @@ int mfc_ens_sync(mfc_ens* e) {
     const int rc = mfc_sync(mb.box);
     if (rc) return efail(e, rc, mfc_last_error(mb.box));
   }
+  return MFC_OK;
+}
+
+int mfc_ens_set_graph(mfc_ens* e, int32_t on) {
+  if (!e) return MFC_ERR_ARG;
+  EHIP(hipStreamSynchronize(e->stream));
+  e->use_graph = on ? 1 : 0;
   return MFC_OK;
 }
 

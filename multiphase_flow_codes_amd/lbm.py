@@ -17,7 +17,8 @@ import numpy as np
 from .params import LBMParams
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmfc_lbm.so")
+# MFC_LIB: another build of the library (e.g. a variant under test)
+LIB_PATH = os.environ.get("MFC_LIB") or os.path.join(HERE, "libmfc_lbm.so")
 
 MFC_OK = 0
 ERRORS = {1: "MFC_ERR_ARG", 2: "MFC_ERR_HIP", 3: "MFC_ERR_STATE", 4: "MFC_ERR_OVERLAP",
@@ -111,6 +112,7 @@ SIGNATURES = {
     "mfc_ens_step": [C.c_void_p, C.c_int32],
     "mfc_ens_sync": [C.c_void_p],
     "mfc_ens_set_block": [C.c_void_p, C.c_int32],
+    "mfc_ens_set_graph": [C.c_void_p, C.c_int32],
     "mfc_ens_autotune": [C.c_void_p, C.c_int32, _I, _D],
     "mfc_ens_get_stream": [C.c_void_p],
 }
@@ -380,6 +382,10 @@ class Ensemble:
 
     def set_block(self, bs: int):
         self._chk(lib().mfc_ens_set_block(self.h, int(bs)))
+
+    def set_graph(self, on: bool):
+        """Replay each step as a recorded HIP graph (see mfc_ens_set_graph)."""
+        self._chk(lib().mfc_ens_set_graph(self.h, int(bool(on))))
 
     def autotune(self, steps: int = 5):
         """Times collide blocks of 64, 128 and 256 threads on the current
