@@ -27,9 +27,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {"fp32": 152, "fp32x": 152, "fp64": 304}   # 19 populations read + 19 written
 # HBM bytes per k_collide dispatch, measured with rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE on this workload (FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0, the
-# factors a plain 1 GiB device copy gives): profiles/hbm_traffic_config2_256cube_300spheres.md
-MEASURED_TRAFFIC = {("config2", "fp32", "ab"): 2.60276e9, ("config2", "fp64", "ab"): 5.15306e9}
+# WRITE_SIZE on these workloads (FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0, the
+# factors a plain 1 GiB device copy gives): profiles/hbm_traffic_config2_config3.md
+# (in-place storage: the mean of its two alternating layouts)
+MEASURED_TRAFFIC = {("config2", "fp32", "ab"): 2.65084e9, ("config2", "fp64", "ab"): 5.20217e9,
+                    ("config3", "fp32", "ab"): 1.70761e11, ("config3", "fp64", "aa"): 3.29957e11}
 
 
 def log(msg):
@@ -128,6 +130,8 @@ def run_ensemble_bench(ncases, steps):
                         f"solid fraction <= 10%, density ratio U[1.5,10], x body force log-U[1e-6,1e-4], "
                         f"nu=0.01, from rest, {steps} steps",
             "cells": int(sum(c.cells for c in cases)), "spheres": int(sum(c.npart for c in cases)),
+            "step_launch": "one recorded HIP graph per step (table upload + 8 stage grids)"
+                           if r32.get("graph") else "8 stage grids per step",
             "fp32": {"mlups": round(r32["mlups"], 1), "ms_per_step": round(r32["ms_per_step"], 4),
                      "collide_block": r32["block"],
                      "autotune_ms_per_step": {str(k): round(v, 4) for k, v in r32["tune_ms_per_step"].items()}},
@@ -326,7 +330,8 @@ def main():
                            ("fp32",) if args.no_fp32x else ("fp32", "fp32x"))
     ens = None if args.no_ensemble else run_ensemble_bench(args.ens_cases, args.ens_steps)
     srv = None if args.no_serving else run_serving_bench(args.serve_cases, args.serve_cap, args.serve_rate)
-    c3 = run_config3(args.config3_steps, min(args.warmup, 10)) if args.config3_steps > 0 else None
+    c3 = (run_config3(args.config3_steps, min(args.warmup, 10), args.config3_fp32_aa, not args.config3_no_fp64)
+          if args.config3_steps > 0 else None)
     cpu = None
     if not args.no_cpu:
         cpu = run_cpu_baseline(case, field, args.cpu_steps)

@@ -315,7 +315,7 @@ int mfc_create(const mfc_params* p, mfc_ctx** out) {
   c.tsize = c.prec == MFC_PREC_FP64 ? 8 : 4;
   {
     const char* e = getenv("MFC_XCD_REMAP");
-    c.xcd_remap = (e && e[0] == '0') ? 0 : 1;
+    c.xcd_remap = (e && e[0] >= '0' && e[0] <= '2') ? e[0] - '0' : -1;
   }
   c.iseedf = p->iseedf;
   int rc = MFC_OK;

@@ -120,7 +120,9 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
     int pl;
     uint32_t src;
     src_of(i, pl, src);
-    f[i] = LAY == LAY_AB ? (R)A[(uint64_t)pl * g.ncell + src] : (R)F[(uint64_t)pl * g.ncell + src];
+    // two buffers: the field read is not read again this step (non-temporal)
+    f[i] = LAY == LAY_AB ? (R)__builtin_nontemporal_load(&A[(uint64_t)pl * g.ncell + src])
+                         : (R)F[(uint64_t)pl * g.ncell + src];
   }
   const bool solid = mask != nullptr && ((mask[n >> 5] >> (n & 31u)) & 1u);
   if (!solid) {
@@ -178,21 +180,41 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
 }
 
 // Workgroups are dealt to the XCDs round-robin in launch order, and each XCD
-// has its own L2.  Tile t (x-chunk fastest, then y, then z) of the box is
-// given to launch index L so that each XCD takes a contiguous range of tiles:
-// the x-chunks of a row, whose loads and stores share cache lines at the
-// chunk edges, and the neighbouring rows run on one XCD.
+// has its own L2.  The tile (x-chunk bx, row) of launch index L:
+//   remap 0: L = bx + nbx * row (x-chunk fastest);
+//   remap 1: each XCD takes a contiguous range of tiles (rows and their chunks);
+//   remap 2: rows stay dealt round-robin (row r on XCD r % 8, so the XCDs
+//            sweep neighbouring addresses together) but all x-chunks of a
+//            row run on the row's XCD, whose L2 then holds the cache lines
+//            the chunks share at their edges.
 constexpr uint32_t kXcds = 8;
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t L, uint32_t total) {
-  const uint32_t per = total / kXcds;
-  if (L >= per * kXcds) return L;   // the last total % 8 tiles keep their index
-  return (L % kXcds) * per + L / kXcds;
+__device__ __forceinline__ void xcd_tile(uint32_t L, uint32_t total, uint32_t nbx, int remap,
+                                         uint32_t& bx, uint32_t& row) {
+  if (remap == 1) {
+    const uint32_t per = total / kXcds;
+    const uint32_t t = L >= per * kXcds ? L : (L % kXcds) * per + L / kXcds;
+    bx = t % nbx;
+    row = t / nbx;
+    return;
+  }
+  if (remap == 2 && nbx > 1) {
+    const uint32_t group = kXcds * nbx;          // 8 rows, all their chunks
+    const uint32_t full = (total / group) * group;
+    if (L < full) {
+      const uint32_t q = L / group, r = L - q * group;
+      bx = r / kXcds;
+      row = q * kXcds + r % kXcds;
+      return;
+    }
+  }
+  bx = L % nbx;
+  row = L / nbx;
 }
 
 template <typename T, typename R, bool ARR, bool WALLS, int LAY>
 __global__ void __launch_bounds__(256) k_collide(CollideArgs a, int nbx, int remap) {
-  const uint32_t t = remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
-  const uint32_t bx = t % (uint32_t)nbx, row = t / (uint32_t)nbx;
+  uint32_t bx, row;
+  xcd_tile(blockIdx.x, gridDim.x, (uint32_t)nbx, remap, bx, row);
   const int x = (int)(bx * blockDim.x + threadIdx.x);
   if (x >= a.g.nx) return;
   const uint32_t y = row % (uint32_t)a.g.ny, z = row / (uint32_t)a.g.ny;
@@ -493,7 +515,9 @@ static void collide_dispatch_lay(Ctx& c, bool arrays, const CollideArgs& a) {
   dim3 grid, blk;
   int nbx;
   row_launch(c.g, grid, blk, nbx);
-  const int rm = c.xcd_remap;
+  // rows split over several chunks (nx > 256): contiguous tile ranges per
+  // XCD; whole-row tiles: launch order (measured best for each)
+  const int rm = c.xcd_remap >= 0 ? c.xcd_remap : (nbx > 1 ? 1 : 0);
   if (arrays) {
     if (c.g.walls_y)
       hipLaunchKernelGGL((k_collide<T, R, true, true, LAY>), grid, blk, 0, c.stream, a, nbx, rm);

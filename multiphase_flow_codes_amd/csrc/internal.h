@@ -153,7 +153,7 @@ struct Ctx {
   double2* diag_Z = nullptr;        // complex FP64 field
   double2* diag_tw[3] = {nullptr, nullptr, nullptr};   // exp(-2 pi i j / n) per axis
   void* ens = nullptr;              // the ensemble this box is a member of (its stream is on loan)
-  int xcd_remap = 1;                // collide tiles grouped per XCD (MFC_XCD_REMAP=0 turns it off)
+  int xcd_remap = -1;               // collide tile order over the XCDs (-1: by shape; MFC_XCD_REMAP=0/1/2)
 };
 
 // ---- launchers (fluid.hip) ----

@@ -147,16 +147,17 @@ class Server:
     """Continuous batching over one Ensemble."""
 
     def __init__(self, precision="fp32", budget_bytes=240e9, window=500, tol=1e-4, cap=20000,
-                 chunk=100, block=None):
+                 chunk=100, block=None, graph=True):
         assert window % chunk == 0 and cap % chunk == 0
         self.precision, self.budget, self.window, self.tol = precision, budget_bytes, window, tol
-        self.cap, self.chunk, self.block = cap, chunk, block
+        self.cap, self.chunk, self.block, self.graph = cap, chunk, block, graph
 
     def run(self, cases: list, clock=time.perf_counter) -> dict:
         pending = sorted(cases, key=lambda c: c.arrival)
         active = {}                  # Lattice -> CaseRecord
         done = []
         ens = Ensemble()
+        ens.set_graph(self.graph)
         if self.block:
             ens.set_block(self.block)
         used = 0
@@ -257,12 +258,13 @@ def summarize(recs: list, wall: float, cells_steps: int) -> dict:
 
 
 def run_ensemble(cases: list, steps: int, precision="fp32", tune_steps=5, sample_every=10,
-                 avg_window=500, clock=time.perf_counter) -> dict:
+                 avg_window=500, graph=True, clock=time.perf_counter) -> dict:
     """All cases in one ensemble for `steps` steps (the autotuner's steps
     included); per-case mean drag averaged over the last `avg_window` steps,
     sampled every `sample_every` steps."""
     lats = [make_lattice(c, precision) for c in cases]
     ens = Ensemble()
+    ens.set_graph(graph)
     try:
         for lat in lats:
             ens.add(lat, 1)
@@ -283,7 +285,7 @@ def run_ensemble(cases: list, steps: int, precision="fp32", tune_steps=5, sample
         ens.sync()
         wall = clock() - t0
         cells = sum(c.cells for c in cases)
-        return {"block": block, "tune_ms_per_step": tune, "steps": steps,
+        return {"block": block, "tune_ms_per_step": tune, "steps": steps, "graph": bool(graph),
                 "ms_per_step": 1e3 * wall / max(done - first, 1),
                 "mlups": cells * (done - first) / wall / 1e6 if wall > 0 else 0.0,
                 "mean_drag": acc / max(nacc, 1)}
