@@ -103,6 +103,7 @@ def run_accuracy(case, field, nsteps, modes=("fp32",)):
         ek_rel = np.abs(a["ek"] - b["ek"]) / np.maximum(b["ek"], 1e-300)
         fn = np.linalg.norm(b["f"], axis=1)
         dfn = np.linalg.norm(a["f"] - b["f"], axis=1)
+        frms = float(np.sqrt(np.mean(fn ** 2)))
         res[mode] = {
             "steps": nsteps, "reference": "plain FP64 GPU path",
             "velocity_rel_l2": du,
@@ -110,6 +111,8 @@ def run_accuracy(case, field, nsteps, modes=("fp32",)):
             "spectrum_energy_weighted_rel": float(np.abs(a["ek"] - b["ek"]).sum() / b["ek"].sum()),
             "drag_rel_l2": float(np.linalg.norm(a["f"] - b["f"]) / np.linalg.norm(b["f"])),
             "drag_max_rel": float((dfn / np.maximum(fn, 1e-300)).max()),
+            "drag_max_err_over_rms_drag": float(dfn.max() / frms),
+            "drag_median_rel": float(np.median(dfn / np.maximum(fn, 1e-300))),
             "kinetic_energy_rel": float(abs(a["sums"]["u2"] - b["sums"]["u2"]) / b["sums"]["u2"])}
         log(f"accuracy {mode}: {res[mode]}")
     return res
@@ -275,8 +278,8 @@ def main():
     ap.add_argument("--ens-cases", type=int, default=96)
     ap.add_argument("--ens-steps", type=int, default=2000)
     ap.add_argument("--serve-cases", type=int, default=24)
-    ap.add_argument("--serve-cap", type=int, default=4000,
-                    help="step cap per served case (the full workload uses 20000)")
+    ap.add_argument("--serve-cap", type=int, default=20000,
+                    help="step cap per served case")
     ap.add_argument("--serve-rate", type=float, default=8.0, help="arrivals per second")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-serving", action="store_true")

@@ -1378,9 +1378,21 @@ __global__ void __launch_bounds__(PB)
   bb_contact_body<T>((int)blockIdx.x - off[c], a);
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin in launch order.  Launch
+// index L of the first nsegs blocks is mapped so that all segments of a
+// particle run on one XCD (particle p on XCD p % 8): neighbouring segments
+// read overlapping stencils and that XCD's L2 then holds them once.
+__device__ __forceinline__ int xcd_segment(int L, int nsegs, int nseg) {
+  const int group = 8 * nseg;
+  const int full = (nsegs / group) * group;
+  if (L >= full) return L;
+  const int q = L / group, r = L - q * group;
+  return (q * 8 + r % 8) * nseg + r / 8;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(PB) k_bounce_back(BounceArgs a) {
-  bounce_back_body<T>((int)blockIdx.x, a);
+  bounce_back_body<T>(xcd_segment((int)blockIdx.x, a.nsegs, a.nseg), a);
 }
 template <typename T>
 __global__ void __launch_bounds__(PB)

@@ -273,6 +273,22 @@ class Lattice:
         self._chk(lib().mfc_rhoupdat(self.h, C.byref(d)))
         return d.value
 
+    def prerelax(self, rhoepsl: float, max_iter: int = 15000) -> tuple:
+        """Pre-relaxation of the density at frozen velocity (BOX/main.f90:86-108):
+        collision_MRT, streaming, rhoupdat until max|rho - rho_prev| <= rhoepsl or
+        the iteration count passes max_iter; then macrovar.  Returns
+        (iterations, final residual)."""
+        it = 0
+        while True:
+            self.collision_mrt()
+            self.streaming()
+            err = self.rhoupdat()
+            if err <= rhoepsl or it > max_iter:
+                break
+            it += 1
+        self.macrovar()
+        return it, err
+
     # -- particles ---------------------------------------------------------------
     def prt_init(self, npart: int):
         self._chk(lib().mfc_prt_init(self.h, int(npart)))

@@ -65,6 +65,30 @@ def test_prerelaxation_array_mode_matches_oracle(oracle_lib):
         assert rel(lat.download_f(), orc.f) < TOL64
 
 
+def test_prerelax_helper_converges_like_the_loop(oracle_lib):
+    """Lattice.prerelax runs the same loop to the residual threshold: the
+    iteration count and field equal the oracle's loop with the same exit."""
+    prm = small_params(forcing=False)
+    lat, orc = _pair(prm)
+    rng = np.random.default_rng(6)
+    start_both(prm, lat, orc, rng, amp=0.03)
+    eps = 1e-7
+    it_o = 0
+    while True:
+        rhop = orc.rho.copy()
+        orc.collision_mrt()
+        orc.streaming()
+        orc.rhoupdat()
+        err_o = np.max(np.abs(orc.rho - rhop))
+        if err_o <= eps or it_o > 15000:
+            break
+        it_o += 1
+    it_g, err_g = lat.prerelax(eps)
+    assert it_g == it_o and it_o > 3
+    assert abs(err_g - err_o) <= 1e-6 * err_o
+    assert rel(lat.download_f(), orc.f) < 1e-11
+
+
 def test_fused_step_equals_fine_grained(oracle_lib):
     from multiphase_flow_codes_amd.lbm import Lattice
     prm = small_params()
