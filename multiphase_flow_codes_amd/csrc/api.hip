@@ -314,6 +314,10 @@ int mfc_create(const mfc_params* p, mfc_ctx** out) {
   c.prec = p->precision;
   c.tsize = c.prec == MFC_PREC_FP64 ? 8 : 4;
   {
+    const char* s = getenv("MFC_BB_SPLIT");
+    if (s && s[0] >= '1' && s[0] <= '0' + kMaxBBSplit) c.pt.bb_split = s[0] - '0';
+  }
+  {
     const char* e = getenv("MFC_XCD_REMAP");
     c.xcd_remap = (e && e[0] >= '0' && e[0] <= '2') ? e[0] - '0' : -1;
   }
@@ -707,7 +711,7 @@ int mfc_prt_init(mfc_ctx* ctx, int32_t npart) {
     HIPCHK(hipMalloc(&P.sh_node, sizeof(uint32_t) * nodes));
     HIPCHK(hipMalloc(&P.sh_hits, sizeof(uint32_t) * nodes));
     HIPCHK(hipMalloc(&P.sh_cnt, sizeof(int32_t) * nsegs));
-    HIPCHK(hipMalloc(&P.seg_ft, sizeof(double) * 6 * nsegs));
+    HIPCHK(hipMalloc(&P.seg_ft, sizeof(double) * 6 * nsegs * kMaxBBSplit));
     // every refilled node was solid: at most the solid nodes of all spheres
     const double rs = c.p.rad + 1.0;
     P.fill_cap = (int64_t)ceil(4.0 / 3.0 * M_PI * rs * rs * rs) * npart + 64;

@@ -40,6 +40,8 @@ constexpr int kCellMinParticles = 512;
 // segment-count scans of more than kScanChunk segments run over several
 // workgroups, kScanChunk segments each
 constexpr int kScanChunk = 4096;
+// a segment's links are split over up to this many bounce-back workgroups
+constexpr int kMaxBBSplit = 4;
 constexpr int kCellCap = 16;
 // The live forcing modes of one FORCING call, grouped by kx (n[0] modes with
 // kx = 0 first, then kx = 1, then kx = 2), each in (kz, ky) order, with
@@ -100,7 +102,11 @@ struct Particles {
   int32_t* lk_cnt = nullptr;   // links per segment
   int64_t* lk_part = nullptr;  // per-chunk sums of the segment counts (multi-block scan)
   int64_t* lk_off = nullptr;   // link offset per segment (np * nseg + 1)
-  double* seg_ft = nullptr;    // force and torque per segment (6 each)
+  double* seg_ft = nullptr;    // force and torque per bounce-back unit (6 each)
+  // bounce-back units per segment (1..kMaxBBSplit, MFC_BB_SPLIT); one
+  // measured best on MI355X at 256^3 and 1024^3 (more units: more blocks,
+  // no better cache reuse)
+  int bb_split = 1;
   // cell grid of the pair searches (on for many particles): per-cell counts
   // and particle lists, valid for the positions at the last build
   int cell_n[3] = {0, 0, 0};

@@ -846,25 +846,29 @@ __device__ __forceinline__ void bb_contact_body(int b_, const BounceArgs& a) {
   }
 }
 
-// beads_collision (partlib.f90:1363-1703), one segment of links per block:
-// the bounce-back value of every link, written at once into the opposite
-// population of the solid neighbour x + c (partlib.f90:1605-1621), and the
-// segment's momentum-exchange force and torque (FP64, fixed reduction order)
-// into seg_ft.  Blocks past the link segments (fused step) reset one segment
+// beads_collision (partlib.f90:1363-1703), one unit (1/split of a segment's
+// links) per block: the bounce-back value of every link, written at once into
+// the opposite population of the solid neighbour x + c (partlib.f90:1605-1621),
+// and the unit's momentum-exchange force and torque (FP64, fixed reduction
+// order) into seg_ft.  Blocks past the units (fused step) reset one segment
 // of the map buffer the coming beads_links rebuilds (built two steps ago).
 template <typename T>
 __device__ __forceinline__ void bounce_back_body(int b_, const BounceArgs& a) {
   __shared__ double red[6][PB / 64];
   T* A = (T*)a.A;
   const Grid& g = a.g;
-  const int b = b_;
-  if (b >= a.nsegs) {
-    clear_segment(b - a.nsegs, a.nseg, a.seg_cap, a.clr_list, a.clr_cnt, a.clr_isn, a.clr_mask);
+  const int u = b_;
+  const int nunits = a.nsegs * a.split;
+  if (u >= nunits) {
+    clear_segment(u - nunits, a.nseg, a.seg_cap, a.clr_list, a.clr_cnt, a.clr_isn, a.clr_mask);
     return;
   }
+  const int b = u / a.split, h = u - b * a.split;
   const int p = b / a.nseg;
-  const int64_t l0 = a.off[b];
-  const int64_t e = min(a.off[b + 1], a.cap);
+  const int64_t s0 = a.off[b], s1 = min(a.off[b + 1], a.cap);
+  const int64_t per = (max(s1 - s0, (int64_t)0) + a.split - 1) / a.split;
+  const int64_t l0 = s0 + h * per;
+  const int64_t e = min(s1, l0 + per);
   const double xcp = a.ypglb[3 * p], ycp = a.ypglb[3 * p + 1], zcp = a.ypglb[3 * p + 2];
   const double w1 = a.wp[3 * p], w2 = a.wp[3 * p + 1], w3 = a.wp[3 * p + 2];
   const double o1 = a.omgp[3 * p], o2 = a.omgp[3 * p + 1], o3 = a.omgp[3 * p + 2];
@@ -924,20 +928,21 @@ __device__ __forceinline__ void bounce_back_body(int b_, const BounceArgs& a) {
     double v = 0.0;
 #pragma unroll
     for (int q = 0; q < PB / 64; ++q) v += red[threadIdx.x][q];
-    a.seg_ft[6ll * b + threadIdx.x] = v;
+    a.seg_ft[6ll * u + threadIdx.x] = v;
   }
   if (contact) atomicAdd(&a.status[ST_CONTACT_LINKS], contact);
 }
 
-// fHIp / torqp of each particle: its segments' sums, in segment order.
-__device__ __forceinline__ void seg_sum(const double* __restrict__ seg_ft, int nseg, int p,
+// fHIp / torqp of each particle: its units' sums (nunit per particle), in
+// unit order.
+__device__ __forceinline__ void seg_sum(const double* __restrict__ seg_ft, int nunit, int p,
                                         double (&f)[3], double (&t)[3]) {
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     double v = 0.0, u = 0.0;
-    for (int s = 0; s < nseg; ++s) {
-      v += seg_ft[6ll * (p * nseg + s) + c];
-      u += seg_ft[6ll * (p * nseg + s) + 3 + c];
+    for (int s = 0; s < nunit; ++s) {
+      v += seg_ft[6ll * ((int64_t)p * nunit + s) + c];
+      u += seg_ft[6ll * ((int64_t)p * nunit + s) + 3 + c];
     }
     f[c] = v;
     t[c] = u;
@@ -1392,7 +1397,7 @@ __device__ __forceinline__ int xcd_segment(int L, int nsegs, int nseg) {
 
 template <typename T>
 __global__ void __launch_bounds__(PB) k_bounce_back(BounceArgs a) {
-  bounce_back_body<T>(xcd_segment((int)blockIdx.x, a.nsegs, a.nseg), a);
+  bounce_back_body<T>(xcd_segment((int)blockIdx.x, a.nsegs * a.split, a.nseg * a.split), a);
 }
 template <typename T>
 __global__ void __launch_bounds__(PB)
@@ -1630,6 +1635,7 @@ BounceArgs bounce_args(Ctx& c, bool fused, int nct, int* bb_blocks, int* pre_blo
   a.ypglb = P.ypglb; a.wp = P.wp; a.omgp = P.omgp;
   a.ww1 = c.p.ww1; a.ww2 = c.p.ww2; a.seg_ft = P.seg_ft; a.status = c.d_status;
   a.nsegs = P.np * P.nseg; a.np = P.np; a.lub = lub_args(c);
+  a.split = P.bb_split;
   a.nct = nct;
   a.nlub = fused ? lub_blocks(c) : 0;
   const int t = P.map_cur ^ 1;
@@ -1637,7 +1643,7 @@ BounceArgs bounce_args(Ctx& c, bool fused, int nct, int* bb_blocks, int* pre_blo
   a.seg_cap = P.seg_cap;
   a.clr_list = P.sol_list[t]; a.clr_cnt = P.sol_cnt[t]; a.clr_isn = P.isn[t]; a.clr_mask = P.mask[t];
   if (nclr) P.map_np[t] = 0;
-  *bb_blocks = a.nsegs + nclr;
+  *bb_blocks = a.nsegs * a.split + nclr;
   *pre_blocks = a.nct + a.nlub;
   return a;
 }
@@ -1655,7 +1661,7 @@ MoveKArgs move_args(Ctx& c, int first, int wrap, bool from_seg, int* blocks) {
   m.bfx = c.p.body_force[0]; m.bfy = c.p.body_force[1]; m.bfz = c.p.body_force[2];
   m.pfx = c.p.particle_force[0]; m.pfy = c.p.particle_force[1]; m.pfz = c.p.particle_force[2];
   a.np = P.np; a.first = first; a.wrap = wrap; a.G = pgeom(c); a.status = c.d_status;
-  a.seg_ft = from_seg ? P.seg_ft : nullptr; a.nseg = P.nseg;
+  a.seg_ft = from_seg ? P.seg_ft : nullptr; a.nseg = P.nseg * P.bb_split;
   *blocks = (P.np + 255) / 256;
   return a;
 }
@@ -1714,8 +1720,8 @@ hipError_t launch_bounce_back(Ctx& c, bool fused) {
     hipLaunchKernelGGL(k_bounce_back<float>, dim3(nb), dim3(PB), 0, c.stream, ba);
   }
   if (!fused)
-    hipLaunchKernelGGL(k_seg_reduce, dim3((P.np + 255) / 256), dim3(256), 0, c.stream, P.seg_ft, P.nseg,
-                       P.np, P.fHIp, P.torqp);
+    hipLaunchKernelGGL(k_seg_reduce, dim3((P.np + 255) / 256), dim3(256), 0, c.stream, P.seg_ft,
+                       P.nseg * P.bb_split, P.np, P.fHIp, P.torqp);
   return hipGetLastError();
 }
 

@@ -153,6 +153,7 @@ struct BounceArgs {
   int32_t* status;
   int nsegs;
   int np;
+  int split;                    // bounce-back work units per segment (seg_ft has nsegs * split)
   int nct;                      // contact blocks of the pre-pass grid
   int nlub;                     // lubrication blocks of the pre-pass grid (fused step, else 0)
   LubArgs lub;
