@@ -254,6 +254,7 @@ def summarize(recs: list, wall: float, cells_steps: int) -> dict:
         "converged": int(sum(r.converged for r in recs)),
         "steps": [r.steps for r in recs],
         "drag": {r.case.seed: r.drag for r in recs},
+        "drag_history": {r.case.seed: list(r.drags) for r in recs},   # every `window` steps
     }
 
 
