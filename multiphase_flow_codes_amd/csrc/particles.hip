@@ -786,27 +786,24 @@ __device__ __forceinline__ void link_bounce(const T* A, const Grid& g, const PGe
   const bool out2 = g.walls_y && (y2 < 0 || y2 >= g.ny);
   const uint32_t m1 = out1 ? n : (uint32_t)wrap1(x - ix, g.nx) + (uint32_t)g.nx * ((uint32_t)wrap1(y1, g.ny) + (uint32_t)g.ny * (uint32_t)wrap1(z - iz, g.nz));
   const uint32_t m2 = out2 ? n : (uint32_t)wrap1(x - 2 * ix, g.nx) + (uint32_t)g.nx * ((uint32_t)wrap1(y2, g.ny) + (uint32_t)g.ny * (uint32_t)wrap1(z - 2 * iz, g.nz));
-  // only the values the interpolation uses are gathered: simple bounce-back
-  // needs f_q(x); the alpha <= 1/2 branch f_q at x - c and x - 2c, the
-  // alpha > 1/2 branch f_qbar at x and x - c
-  const bool sol1 = out1 || bit_at(mask, m1);
+  // the values the interpolation may use are gathered together with the
+  // solidity tests of x - c and x - 2c (one dependent load stage less; a
+  // value at a solid node is read but not used): the alpha <= 1/2 branch
+  // takes f_q at x, x - c and x - 2c, the alpha > 1/2 branch f_q at x and
+  // f_qbar at x and x - c
   const int qo = opp(q);
   const double wt = q <= 6 ? ww1 : ww2;
+  const bool lo = alpha <= 0.5;
   ff1 = post_value(A, g, q, n);
-  if (alpha <= 0.5) {
-    bool sol2 = true;
-    double ff2a = 0.0, ff3a = 0.0;
-    if (!sol1) {
-      sol2 = out2 || bit_at(mask, m2);
-      ff2a = post_value(A, g, q, m1);
-      if (!sol2) ff3a = post_value(A, g, q, m2);
-    }
-    f9 = bounce_back_value(alpha, ff1, ff2a, ff3a, sol1, sol2, 0.0, 0.0, wt, uwpro);
-  } else {
-    const double ffo = post_value(A, g, qo, n);
-    const double ffo1 = sol1 ? 0.0 : post_value(A, g, qo, m1);
-    f9 = bounce_back_value(alpha, ff1, 0.0, 0.0, sol1, true, ffo, ffo1, wt, uwpro);
-  }
+  const double v1 = post_value(A, g, lo ? q : qo, lo ? m1 : n);
+  const double v2 = post_value(A, g, lo ? q : qo, lo ? m2 : m1);
+  const bool sol1 = out1 || bit_at(mask, m1);
+  const bool sol2 = out2 || bit_at(mask, m2);
+  if (lo)
+    f9 = bounce_back_value(alpha, ff1, sol1 ? 0.0 : v1, (sol1 || sol2) ? 0.0 : v2, sol1, sol1 || sol2,
+                           0.0, 0.0, wt, uwpro);
+  else
+    f9 = bounce_back_value(alpha, ff1, 0.0, 0.0, sol1, true, v1, sol1 ? 0.0 : v2, wt, uwpro);
 }
 
 // Pre-pass of the bounce-back.  Contact links (node x solid for another
