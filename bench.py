@@ -3,7 +3,7 @@
 Workload (one "step" = one pass of the reference time loop body,
 BOX/main.f90:172-188: FORCING, collision_MRT, beads_collision, streaming,
 beads_lubforce, beads_move, beads_redistribute, beads_links, beads_filling,
-macrovar): config 2, a 256^3 periodic box, nu = 0.0036, forcing on the
+macrovar): box256, a 256^3 periodic box, nu = 0.0036, forcing on the
 |k| <= sqrt(2) shell, 300 spheres of radius 8 (density ratio 5), synthetic
 solenoidal initial field (k0 = 4, u_rms = 0.03).  State is resident in HBM
 before the timed region starts.
@@ -28,10 +28,10 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {"fp32": 152, "fp32x": 152, "fp64": 304}   # 19 populations read + 19 written
 # HBM bytes per k_collide dispatch, measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on these workloads (FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0, the
-# factors a plain 1 GiB device copy gives): profiles/hbm_traffic_config2_config3.md
+# factors a plain 1 GiB device copy gives): profiles/hbm_traffic_box256_channel1024.md
 # (in-place storage: the mean of its two alternating layouts)
-MEASURED_TRAFFIC = {("config2", "fp32", "ab"): 2.65084e9, ("config2", "fp64", "ab"): 5.20217e9,
-                    ("config3", "fp32", "ab"): 1.70761e11, ("config3", "fp64", "aa"): 3.29957e11}
+MEASURED_TRAFFIC = {("box256", "fp32", "ab"): 2.65084e9, ("box256", "fp64", "ab"): 5.20217e9,
+                    ("channel1024", "fp32", "ab"): 1.70761e11, ("channel1024", "fp64", "aa"): 3.29957e11}
 
 
 def log(msg):
@@ -175,22 +175,22 @@ def run_serving_bench(ncases, cap, rate, window=500, tol=1e-4):
             "cases_stopped_at_other_step": differ}
 
 
-def run_config3(steps, warmup, fp32_aa=False, fp64=True):
+def run_channel(steps, warmup, fp32_aa=False, fp64=True):
     """The 1024^3 channel with 20,000 spheres: FP32 shifted populations with
     two buffers (163 GB) beside FP64 with one buffer updated in place
     (AA pattern, 163 GB); two FP64 buffers would need 326 GB."""
     from multiphase_flow_codes_amd import workloads
-    case = workloads.CONFIG3
+    case = workloads.CHANNEL1024
     keep = ("mlups", "ms_per_step", "kernel_ms", "roofline", "links")
-    r32, _ = run_gpu(case, "fp32", steps, warmup, None, "config3")
+    r32, _ = run_gpu(case, "fp32", steps, warmup, None, "channel1024")
     out = {"workload": case.describe(), "cells": case.n ** 3, "particles": case.npart,
            "steps": steps, "warmup": warmup,
            "fp32": {k: r32[k] for k in keep} | {"storage": "two buffers (A/B), 163 GB"}}
     if fp32_aa:
-        ra, _ = run_gpu(case, "fp32", steps, warmup, None, "config3", storage="aa")
+        ra, _ = run_gpu(case, "fp32", steps, warmup, None, "channel1024", storage="aa")
         out["fp32_aa"] = {k: ra[k] for k in keep} | {"storage": "one buffer in place (AA), 82 GB"}
     if fp64:
-        r64, _ = run_gpu(case, "fp64", steps, warmup, None, "config3", storage="aa")
+        r64, _ = run_gpu(case, "fp64", steps, warmup, None, "channel1024", storage="aa")
         out["fp64"] = {k: r64[k] for k in keep} | {"storage": "one buffer in place (AA), 163 GB"}
         out["fp32_over_fp64"] = round(r32["mlups"] / r64["mlups"], 3)
     return out
@@ -276,7 +276,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--case", default="config2", choices=["config1", "config2"])
+    ap.add_argument("--case", default="box256", choices=["box64", "box256"])
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--accuracy-steps", type=int, default=1000,
@@ -294,13 +294,13 @@ def main():
                     help="skip the FP32-storage / FP64-collision variant")
     ap.add_argument("--no-fp32", action="store_true",
                     help="skip the FP32 path (profiling runs of the FP64 kernels only)")
-    ap.add_argument("--config3-steps", type=int, default=100,
+    ap.add_argument("--channel-steps", type=int, default=100,
                     help="steps of the 1024^3 channel leg (FP32 A/B and FP64 AA; 0: skip)")
-    ap.add_argument("--config3-only", action="store_true",
+    ap.add_argument("--channel-only", action="store_true",
                     help="run only the 1024^3 channel leg (profiling)")
-    ap.add_argument("--config3-fp32-aa", action="store_true",
+    ap.add_argument("--channel-fp32-aa", action="store_true",
                     help="also run the 1024^3 channel in FP32 with in-place storage")
-    ap.add_argument("--config3-no-fp64", action="store_true")
+    ap.add_argument("--channel-no-fp64", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized buffer 5 times (HBM counter calibration)")
     ap.add_argument("--ceilings", action="store_true",
@@ -318,11 +318,11 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
     from multiphase_flow_codes_amd import workloads
-    if args.config3_only:
-        log(json.dumps(run_config3(args.config3_steps, min(args.warmup, 10), args.config3_fp32_aa,
-                                   not args.config3_no_fp64)))
+    if args.channel_only:
+        log(json.dumps(run_channel(args.channel_steps, min(args.warmup, 10), args.channel_fp32_aa,
+                                   not args.channel_no_fp64)))
         return
-    case = workloads.CONFIG2 if args.case == "config2" else workloads.CONFIG1
+    case = workloads.WORKLOADS[args.case]
     r32 = r64 = field = None
     if not args.no_fp32:
         r32, field = run_gpu(case, "fp32", args.steps, args.warmup, None, args.case)
@@ -340,8 +340,8 @@ def main():
                            ("fp32",) if args.no_fp32x else ("fp32", "fp32x"))
     ens = None if args.no_ensemble else run_ensemble_bench(args.ens_cases, args.ens_steps)
     srv = None if args.no_serving else run_serving_bench(args.serve_cases, args.serve_cap, args.serve_rate)
-    c3 = (run_config3(args.config3_steps, min(args.warmup, 10), args.config3_fp32_aa, not args.config3_no_fp64)
-          if args.config3_steps > 0 else None)
+    c3 = (run_channel(args.channel_steps, min(args.warmup, 10), args.channel_fp32_aa, not args.channel_no_fp64)
+          if args.channel_steps > 0 else None)
     cpu = None
     if not args.no_cpu:
         cpu = run_cpu_baseline(case, field, args.cpu_steps)
@@ -381,7 +381,7 @@ def main():
         out["p50_case_latency_s"] = srv["engine_fp32"]["p50_s"]
         out["p99_case_latency_s"] = srv["engine_fp32"]["p99_s"]
     if c3 is not None:
-        out["config3"] = c3
+        out["channel1024"] = c3
     if acc is not None:
         out["accuracy_fp32_vs_fp64"] = acc["fp32"]
         if "fp32x" in acc:

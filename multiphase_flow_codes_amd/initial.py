@@ -1,6 +1,6 @@
 """Initial-condition generators for the benchmark and test workloads.
 
-``solenoidal_field`` draws the configs' random solenoidal velocity field:
+``solenoidal_field`` draws the benchmark workloads' random solenoidal velocity field:
 Fourier modes 0 < |k| <= kmax with energy spectrum E(k) ~ k^4 exp(-2 (k/k0)^2),
 random phases, divergence-free by projection, scaled to a given u_rms
 (sqrt of the mean of u.u/3).  The reference itself starts its runs from rest

@@ -1,13 +1,14 @@
-"""Benchmark workloads (the configurations the measurements are quoted on).
+"""Benchmark workloads (the cases the measurements are quoted on).
 
-``BoxCase`` builds the boxes with spheres of the configs:
-  config 1: 64^3 periodic, nu = 0.005, 8 spheres r = 4, density ratio 5,
-            k0 = 4, u_rms 0.02, forced
-  config 2: 256^3 periodic, nu = 0.0036, 300 spheres r = 8, density ratio 5,
-            k0 = 4, u_rms 0.03, forced
-  config 3: 1024^3 channel (half-way bounce-back walls on the y faces), decaying
-            (no forcing), nu = 0.002, 20,000 spheres r = 6, density ratio 2.5,
-            k0 = 8, u_rms 0.03, sphere surfaces at least 8 nodes from the walls
+``BoxCase`` builds the boxes with spheres:
+  box64:       64^3 periodic, nu = 0.005, 8 spheres r = 4, density ratio 5,
+               k0 = 4, u_rms 0.02, forced
+  box256:      256^3 periodic, nu = 0.0036, 300 spheres r = 8, density ratio 5,
+               k0 = 4, u_rms 0.03, forced
+  channel1024: 1024^3 channel (half-way bounce-back walls on the y faces),
+               decaying (no forcing), nu = 0.002, 20,000 spheres r = 6,
+               density ratio 2.5, k0 = 8, u_rms 0.03, sphere surfaces at least
+               8 nodes from the walls
 Forcing acts on the |k| <= sqrt(2) shell with the reference's
 Ornstein-Uhlenbeck mode process (gaussian.f90:182-372).  Spheres start from
 rest at seeded non-overlapping positions (min surface gap 2): the reference's
@@ -69,10 +70,10 @@ class BoxCase:
         return solenoidal_field(self.shape, k0=self.k0, u_rms=self.u_rms, seed=IFLOWSEED_REF)
 
 
-CONFIG1 = BoxCase("config1", 64, 0.005, 8, 4.0, 5.0, 4.0, 0.02)
-CONFIG2 = BoxCase("config2", 256, 0.0036, 300, 8.0, 5.0, 4.0, 0.03)
-CONFIG3 = BoxCase("config3", 1024, 0.002, 20000, 6.0, 2.5, 8.0, 0.03, walls=True, wall_clear=8.0)
-CONFIGS = {c.name: c for c in (CONFIG1, CONFIG2, CONFIG3)}
+BOX64 = BoxCase("box64", 64, 0.005, 8, 4.0, 5.0, 4.0, 0.02)
+BOX256 = BoxCase("box256", 256, 0.0036, 300, 8.0, 5.0, 4.0, 0.03)
+CHANNEL1024 = BoxCase("channel1024", 1024, 0.002, 20000, 6.0, 2.5, 8.0, 0.03, walls=True, wall_clear=8.0)
+WORKLOADS = {c.name: c for c in (BOX64, BOX256, CHANNEL1024)}
 
 
 def setup_lattice(case: BoxCase, precision: str, field=None, storage: str = "ab"):

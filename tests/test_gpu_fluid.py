@@ -1,9 +1,8 @@
 """GPU parity of the fluid step (no particles) against the FP64 oracle.
 
-Bar: the FP64 library matches the oracle to <= 1e-12 normwise relative error
-per step (<= 1e-12 relative per step for the plain FP64
-drop-in); the FP32-shifted path is compared with the FP64 path by the
-relative L2 norm of the velocity field.
+Bar: the FP64 library (the drop-in) matches the oracle to <= 1e-12 normwise
+relative error per step; the FP32-shifted path is compared with the FP64 path
+by the relative L2 norm of the velocity field.
 """
 import numpy as np
 import pytest
