@@ -38,7 +38,9 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_gpu(case, precision, steps, warmup, field, case_name, storage="ab"):
+def run_gpu(case, precision, steps, warmup, field, case_name, storage="ab", total_steps=0, collect=None):
+    """Set-up, `warmup` untimed steps, `steps` timed steps; then (accuracy
+    legs) untimed steps up to `total_steps` and collect(lat) of the state."""
     from multiphase_flow_codes_amd.workloads import setup_lattice
     t0 = time.time()
     lat, prm, field = setup_lattice(case, precision, field, storage=storage)
@@ -63,8 +65,17 @@ def run_gpu(case, precision, steps, warmup, field, case_name, storage="ab"):
     achieved = BYTES_PER_CELL[precision] * ncell / (avg_kern * 1e-3) / 1e9
     log(f"{tag}: {ms_step:.3f} ms/step, {mlups:.0f} MLUPS, collide {avg_kern:.3f} ms "
         f"({achieved:.0f} GB/s), links {st['nlinks']}, refilled nodes {st['refill']}")
+    state = None
+    if collect is not None:
+        done = warmup + steps
+        if total_steps > done:
+            lat.step(1 + done, total_steps - done)
+            lat.sync()
+        state = collect(lat)
+        log(f"{tag}: state after {max(total_steps, done)} steps collected")
     lat.close()
     return {
+        "state": state,
         "mlups": mlups, "ms_per_step": ms_step, "kernel_ms": avg_kern, "launches": launches,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -175,14 +186,44 @@ def run_serving_bench(ncases, cap, rate, window=500, tol=1e-4):
             "cases_stopped_at_other_step": differ}
 
 
-def run_channel(steps, warmup, fp32_aa=False, fp64=True):
+def _channel_state(lat):
+    """Velocity (kept in FP32 on the host: 13 GB at 1024^3), the statistc
+    spectrum and the per-sphere hydrodynamic force."""
+    import numpy as np
+    _, ux, uy, uz = lat.download_macro()
+    u = [a.astype(np.float32) for a in (ux, uy, uz)]
+    del ux, uy, uz
+    return {"u": u, "ek": lat.diag_spectrum(lat.prm.nx // 2 - 1), "f": lat.prt_get("fHIp")}
+
+
+def _sq_sum(arrs, other=None, chunk=1 << 26):
+    """sum of squares (or of squared differences) in FP64, in chunks"""
+    import numpy as np
+    s = 0.0
+    for k, a in enumerate(arrs):
+        fa = a.reshape(-1)
+        fb = None if other is None else other[k].reshape(-1)
+        for i in range(0, fa.size, chunk):
+            x = fa[i:i + chunk].astype(np.float64)
+            if fb is not None:
+                x -= fb[i:i + chunk]
+            s += float(x @ x)
+    return s
+
+
+def run_channel(steps, warmup, fp32_aa=False, fp64=True, acc_steps=500):
     """The 1024^3 channel with 20,000 spheres: FP32 shifted populations with
     two buffers (163 GB) beside FP64 with one buffer updated in place
-    (AA pattern, 163 GB); two FP64 buffers would need 326 GB."""
+    (AA pattern, 163 GB); two FP64 buffers would need 326 GB.  With acc_steps
+    both runs continue to that step and the FP32 velocity, spectrum and
+    sphere drag are compared with the FP64 run's."""
+    import numpy as np
     from multiphase_flow_codes_amd import workloads
     case = workloads.CHANNEL1024
     keep = ("mlups", "ms_per_step", "kernel_ms", "roofline", "links")
-    r32, _ = run_gpu(case, "fp32", steps, warmup, None, "channel1024")
+    acc = acc_steps > 0 and fp64
+    col = _channel_state if acc else None
+    r32, _ = run_gpu(case, "fp32", steps, warmup, None, "channel1024", total_steps=acc_steps, collect=col)
     out = {"workload": case.describe(), "cells": case.n ** 3, "particles": case.npart,
            "steps": steps, "warmup": warmup,
            "fp32": {k: r32[k] for k in keep} | {"storage": "two buffers (A/B), 163 GB"}}
@@ -190,9 +231,24 @@ def run_channel(steps, warmup, fp32_aa=False, fp64=True):
         ra, _ = run_gpu(case, "fp32", steps, warmup, None, "channel1024", storage="aa")
         out["fp32_aa"] = {k: ra[k] for k in keep} | {"storage": "one buffer in place (AA), 82 GB"}
     if fp64:
-        r64, _ = run_gpu(case, "fp64", steps, warmup, None, "channel1024", storage="aa")
+        r64, _ = run_gpu(case, "fp64", steps, warmup, None, "channel1024", storage="aa",
+                         total_steps=acc_steps, collect=col)
         out["fp64"] = {k: r64[k] for k in keep} | {"storage": "one buffer in place (AA), 163 GB"}
         out["fp32_over_fp64"] = round(r32["mlups"] / r64["mlups"], 3)
+    if acc:
+        a, b = r32["state"], r64["state"]
+        du = float(np.sqrt(_sq_sum(a["u"], b["u"]) / _sq_sum(b["u"])))
+        fn = np.linalg.norm(b["f"], axis=1)
+        dfn = np.linalg.norm(a["f"] - b["f"], axis=1)
+        ek = np.abs(a["ek"] - b["ek"]) / np.maximum(b["ek"], 1e-300)
+        out["accuracy_fp32_vs_fp64"] = {
+            "steps": acc_steps, "reference": "FP64 run of the same case (in-place storage)",
+            "velocity_rel_l2": du, "note": "velocities compared as FP32 copies (6e-8 relative)",
+            "spectrum_max_rel": float(ek.max()),
+            "drag_rel_l2": float(np.linalg.norm(a["f"] - b["f"]) / np.linalg.norm(b["f"])),
+            "drag_median_rel": float(np.median(dfn / np.maximum(fn, 1e-300))),
+            "drag_max_err_over_rms_drag": float(dfn.max() / np.sqrt(np.mean(fn ** 2)))}
+        log(f"channel accuracy: {out['accuracy_fp32_vs_fp64']}")
     return out
 
 
@@ -296,6 +352,8 @@ def main():
                     help="skip the FP32 path (profiling runs of the FP64 kernels only)")
     ap.add_argument("--channel-steps", type=int, default=100,
                     help="steps of the 1024^3 channel leg (FP32 A/B and FP64 AA; 0: skip)")
+    ap.add_argument("--channel-accuracy-steps", type=int, default=500,
+                    help="run both channel legs to this step and compare FP32 with FP64 (0: skip)")
     ap.add_argument("--channel-only", action="store_true",
                     help="run only the 1024^3 channel leg (profiling)")
     ap.add_argument("--channel-fp32-aa", action="store_true",
@@ -320,7 +378,7 @@ def main():
     from multiphase_flow_codes_amd import workloads
     if args.channel_only:
         log(json.dumps(run_channel(args.channel_steps, min(args.warmup, 10), args.channel_fp32_aa,
-                                   not args.channel_no_fp64)))
+                                   not args.channel_no_fp64, args.channel_accuracy_steps)))
         return
     case = workloads.WORKLOADS[args.case]
     r32 = r64 = field = None
@@ -340,7 +398,8 @@ def main():
                            ("fp32",) if args.no_fp32x else ("fp32", "fp32x"))
     ens = None if args.no_ensemble else run_ensemble_bench(args.ens_cases, args.ens_steps)
     srv = None if args.no_serving else run_serving_bench(args.serve_cases, args.serve_cap, args.serve_rate)
-    c3 = (run_channel(args.channel_steps, min(args.warmup, 10), args.channel_fp32_aa, not args.channel_no_fp64)
+    c3 = (run_channel(args.channel_steps, min(args.warmup, 10), args.channel_fp32_aa, not args.channel_no_fp64,
+                      args.channel_accuracy_steps)
           if args.channel_steps > 0 else None)
     cpu = None
     if not args.no_cpu:
