@@ -49,7 +49,9 @@ def run_gpu(case, precision, steps, warmup, field, case_name, storage="ab", tota
     lat.step(1, warmup)
     lat.sync()
     log(f"{tag}: warm-up {warmup} steps done")
-    lat.set_timing(True)
+    # HIP events around a sample of ~25 collide launches of the timed steps
+    # (each event marker idles the GPU for a few microseconds)
+    lat.set_timing(max(1, steps // 25))
     lat.sync()
     t1 = time.perf_counter()
     lat.step(1 + warmup, steps)

@@ -210,7 +210,13 @@ int mfc_ens_set_graph(mfc_ens* e, int32_t on);
 int mfc_ens_autotune(mfc_ens* e, int32_t steps, int32_t* chosen, double* ms_per_step3);
 void* mfc_ens_get_stream(mfc_ens* e);
 
-int mfc_set_timing(mfc_ctx* ctx, int32_t on);
+/* Collide-kernel timing with HIP events on the library's stream: every = N
+ * brackets every N-th collide launch from now on (the first one included)
+ * with an event pair, 0 turns it off.  Each event marker idles the GPU for a
+ * few microseconds, so a sample (N > 1) keeps that out of the step time.
+ * mfc_get_timing returns the summed duration of the timed launches and their
+ * number. */
+int mfc_set_timing(mfc_ctx* ctx, int32_t every);
 int mfc_get_timing(mfc_ctx* ctx, double* total_ms, int64_t* launches);
 
 #ifdef __cplusplus

@@ -612,7 +612,7 @@ int mfc_collision_mrt(mfc_ctx* ctx) {
   } else {
     return fail(ctx, MFC_ERR_STATE, "collision_MRT: rho/u from macrovar no longer match the field");
   }
-  if (c.timing && c.tpool_used + 2 <= (int)c.tpool.size()) {
+  if (c.timing && c.tcount++ % c.timing == 0 && c.tpool_used + 2 <= (int)c.tpool.size()) {
     HIPCHK(hipEventRecord(c.tpool[c.tpool_used], c.stream));
     HIPCHK(launch_collide(c, arrays));
     HIPCHK(hipEventRecord(c.tpool[c.tpool_used + 1], c.stream));
@@ -1111,7 +1111,8 @@ int mfc_set_timing(mfc_ctx* ctx, int32_t on) {
   }
   HIPCHK(hipStreamSynchronize(c.stream));
   c.tpool_used = 0;
-  c.timing = on ? 1 : 0;
+  c.tcount = 0;
+  c.timing = on > 0 ? on : 0;
   return MFC_OK;
 }
 

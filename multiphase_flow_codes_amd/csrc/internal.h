@@ -147,9 +147,10 @@ struct Ctx {
   void* staging = nullptr;          // device staging for host transfers
   size_t staging_bytes = 0;
   std::string err;
-  // optional per-launch timing of the collide kernel: event pairs recorded
-  // around each launch, summed by mfc_get_timing
+  // optional timing of the collide kernel: event pairs recorded around every
+  // `timing`-th launch (0: off), summed by mfc_get_timing
   int timing = 0;
+  int64_t tcount = 0;
   std::vector<hipEvent_t> tpool;
   int tpool_used = 0;
   // diagnostics scratch, allocated on first use

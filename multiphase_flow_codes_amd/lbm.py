@@ -356,8 +356,10 @@ class Lattice:
         self._chk(lib().mfc_diag_spectrum(self.h, int(nek), _dp(ek)))
         return ek
 
-    def set_timing(self, on: bool):
-        self._chk(lib().mfc_set_timing(self.h, int(bool(on))))
+    def set_timing(self, every):
+        """Time every `every`-th collide launch with HIP events (True: each
+        launch; False or 0: off)."""
+        self._chk(lib().mfc_set_timing(self.h, int(every)))
 
     def get_timing(self):
         t = C.c_double()

@@ -138,3 +138,21 @@ def test_fp32_tracks_fp64(oracle_lib):
     assert np.all(np.isfinite(u64))
     err = np.linalg.norm(u32 - u64) / np.linalg.norm(u64)
     assert err < 1e-5, err
+
+
+def test_sampled_collide_timing(oracle_lib):
+    """mfc_set_timing(every=N) brackets launches 0, N, 2N, ... with events."""
+    from multiphase_flow_codes_amd.lbm import Lattice
+    prm = small_params(n=(32, 32, 32))
+    lat = Lattice(prm)
+    lat.upload_macro(None, *smooth_velocity(prm, np.random.default_rng(4)))
+    lat.initpop()
+    lat.rhoupdat()
+    lat.macrovar()
+    lat.set_timing(3)
+    lat.step(1, 10)
+    ms, n = lat.get_timing()
+    assert n == 4 and ms > 0.0
+    lat.set_timing(0)
+    lat.step(11, 2)
+    assert lat.get_timing() == (0.0, 0)
