@@ -99,30 +99,31 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
   const uint32_t xs[3] = {xpp, (uint32_t)x, xm};   // cx = -1, 0, +1
   const uint32_t n = (uint32_t)x + nx * row;
   const bool wall_lo = WALLS && y == 0, wall_hi = WALLS && y == g.ny - 1;
-  // location of the streamed value of direction i
-  auto src_of = [&](int i, int& pl, uint32_t& src) {
+  // location of the streamed value of direction i: plane offset (pl * ncell,
+  // a select between two compile-time planes on the wave-uniform wall test,
+  // so it stays a scalar) and slot
+  auto src_of = [&](int i, uint64_t& pbase, uint32_t& src) {
     if (LAY == LAY_AA_ODD) {
-      pl = i;
+      pbase = (uint64_t)i * g.ncell;
       src = n;
       return;
     }
     const int ccx = cx(i), ccy = cy(i), ccz = cz(i);
     const bool wall = (ccy > 0 && wall_lo) || (ccy < 0 && wall_hi);
     src = wall ? n : xs[ccx + 1] + nx * (yy[ccy + 1] + ny * zz[ccz + 1]);
-    if (LAY == LAY_AB)
-      pl = wall ? opp(i) : i;
-    else
-      pl = wall ? i : opp(i);
+    const int p_in = LAY == LAY_AB ? i : opp(i);       // plane of an interior pull
+    const int p_wall = LAY == LAY_AB ? opp(i) : i;     // plane beyond a wall
+    const uint64_t b_in = (uint64_t)p_in * g.ncell, b_wall = (uint64_t)p_wall * g.ncell;
+    pbase = wall ? b_wall : b_in;
   };
   R f[NPOP];
 #pragma unroll
   for (int i = 0; i < NPOP; ++i) {
-    int pl;
+    uint64_t pb;
     uint32_t src;
-    src_of(i, pl, src);
+    src_of(i, pb, src);
     // two buffers: the field read is not read again this step (non-temporal)
-    f[i] = LAY == LAY_AB ? (R)__builtin_nontemporal_load(&A[(uint64_t)pl * g.ncell + src])
-                         : (R)F[(uint64_t)pl * g.ncell + src];
+    f[i] = LAY == LAY_AB ? (R)__builtin_nontemporal_load(&A[pb + src]) : (R)F[pb + src];
   }
   const bool solid = mask != nullptr && ((mask[n >> 5] >> (n & 31u)) & 1u);
   if (!solid) {
@@ -166,15 +167,20 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
       mrt_collide<R, false>(f, rho, ux, uy, uz, Fx, Fy, Fz, k);
   }
   if (LAY == LAY_AB) {
+    // one running pointer over the planes: no per-plane 64-bit base to keep
+    T* q = B + n;
 #pragma unroll
-    for (int i = 0; i < NPOP; ++i) __builtin_nontemporal_store((T)f[i], &B[(uint64_t)i * g.ncell + n]);
+    for (int i = 0; i < NPOP; ++i) {
+      __builtin_nontemporal_store((T)f[i], q);
+      q += g.ncell;
+    }
   } else {
 #pragma unroll
     for (int i = 0; i < NPOP; ++i) {
-      int pl;
+      uint64_t pb;
       uint32_t dst;
-      src_of(opp(i), pl, dst);
-      __builtin_nontemporal_store((T)f[i], &F[(uint64_t)pl * g.ncell + dst]);
+      src_of(opp(i), pb, dst);
+      __builtin_nontemporal_store((T)f[i], &F[pb + dst]);
     }
   }
 }
