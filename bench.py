@@ -50,8 +50,10 @@ def run_gpu(case, precision, steps, warmup, field, case_name, storage="ab", tota
     lat.sync()
     log(f"{tag}: warm-up {warmup} steps done")
     # HIP events around a sample of ~25 collide launches of the timed steps
-    # (each event marker idles the GPU for a few microseconds)
-    lat.set_timing(max(1, steps // 25))
+    # (each event marker idles the GPU for a few microseconds); an odd period,
+    # so in-place storage samples both of its alternating layouts
+    every = max(1, steps // 25)
+    lat.set_timing(every if every % 2 else every + 1)
     lat.sync()
     t1 = time.perf_counter()
     lat.step(1 + warmup, steps)
