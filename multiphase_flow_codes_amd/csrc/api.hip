@@ -343,8 +343,6 @@ int mfc_create(const mfc_params* p, mfc_ctx** out) {
   }
   const size_t gb = sizeof(double) * 18 * (size_t)p->ny * p->nz;
   for (int s = 0; s < kForceSlots; ++s) CK(hipMalloc(&c.ft.g[s], gb));
-  if (c.prec == MFC_PREC_FP32)
-    for (int s = 0; s < kForceSlots; ++s) CK(hipMalloc(&c.ft.g32[s], gb / 2));
   c.ft.cur = 0;
   c.ft.mac = 0;
   {
@@ -382,7 +380,6 @@ int mfc_destroy(mfc_ctx* ctx) {
   F(c.pop[0]); F(c.pop[1]);
   F(c.rho); F(c.ux); F(c.uy); F(c.uz);
   for (int s = 0; s < kForceSlots; ++s) F(c.ft.g[s]);
-  for (int s = 0; s < kForceSlots; ++s) F(c.ft.g32[s]);
   F(c.ft.phx);
   Particles& P = c.pt;
   double* pd[] = {P.yp, P.ypglb, P.ypglbp, P.wp, P.wpp, P.omgp, P.omgpp, P.fHIp, P.flubp, P.torqp,

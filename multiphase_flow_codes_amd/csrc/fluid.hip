@@ -40,15 +40,14 @@ __device__ __forceinline__ void force_at(const double* __restrict__ G, const dou
   Fz = v[2];
 }
 
-// Forcing field at node x from one row of a forcing table (FP64, or its FP32
-// copy for FP32 arithmetic), in precision R.
-template <typename R, typename Q>
-__device__ __forceinline__ void force_row(const Q* __restrict__ t, R c1, R s1, R c2, R s2,
+// Forcing field at node x from one row of a forcing table, in precision R.
+template <typename R>
+__device__ __forceinline__ void force_row(const double* __restrict__ t, R c1, R s1, R c2, R s2,
                                           R& Fx, R& Fy, R& Fz) {
   R v[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const Q* q = t + 6 * c;
+    const double* q = t + 6 * c;
     v[c] = (R)q[0] + (R)2 * ((R)q[2] * c1 - (R)q[3] * s1) + (R)2 * ((R)q[4] * c2 - (R)q[5] * s2);
   }
   Fx = v[0];
@@ -76,10 +75,6 @@ __device__ __forceinline__ void force_row(const Q* __restrict__ t, R c1, R s1, R
 // node touches exactly the locations it read and nodes never race.  From
 // LAY_AA_ODD that is all in the node's own slots; from LAY_AA_EVEN it is the
 // neighbours' (the result is in LAY_AA_ODD, and the other way round).
-// The collision constants in the arithmetic type of the kernel.
-__device__ __forceinline__ const CollideConsts& consts_of(const CollideArgs& a, double) { return a.k; }
-__device__ __forceinline__ const CollideConstsF& consts_of(const CollideArgs& a, float) { return a.kf; }
-
 template <typename T, typename R, bool ARR, bool WALLS, int LAY>
 __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y, int z) {
   const T* __restrict__ A = (const T*)a.A;
@@ -87,7 +82,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
   T* F = (T*)a.B;   // the one buffer of the in-place layouts (A == B)
   const uint32_t* __restrict__ mask = a.mask;
   const Grid& g = a.g;
-  const auto& k = consts_of(a, R{});
+  const CollideConsts& k = a.k;
   const ForceArgs& fa = a.fa;
   const double* __restrict__ rho_a = a.rho;
   const double* __restrict__ ux_a = a.ux;
@@ -136,7 +131,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
     const R c1 = (R)fa.phx[2 * x], s1 = (R)fa.phx[2 * x + 1];
     const R c2 = (R)fa.phx[2 * x2], s2 = (R)fa.phx[2 * x2 + 1];
     R rho, ux, uy, uz;
-    R mx = (R)k.bfx, my = (R)k.bfy, mz = (R)k.bfz;   // force of the macrovar moments
+    R mx = (R)fa.bfx, my = (R)fa.bfy, mz = (R)fa.bfz;   // force of the macrovar moments
     if (ARR) {
       rho = (R)rho_a[n];
       ux = (R)ux_a[n];
@@ -147,10 +142,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
       density_momentum<R>(f, rho, jx, jy, jz);
       if (fa.gmac) {
         R gx, gy, gz;
-        if constexpr (sizeof(R) == 4)
-          force_row<R>(fa.gmac32 + 18ull * row, c1, s1, c2, s2, gx, gy, gz);
-        else
-          force_row<R>(fa.gmac + 18ull * row, c1, s1, c2, s2, gx, gy, gz);
+        force_row<R>(fa.gmac + 18ull * row, c1, s1, c2, s2, gx, gy, gz);
         mx += gx;
         my += gy;
         mz += gz;
@@ -159,13 +151,10 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
       uy = jy + (R)0.5 * my;
       uz = jz + (R)0.5 * mz;
     }
-    R Fx = (R)k.bfx, Fy = (R)k.bfy, Fz = (R)k.bfz;
+    R Fx = (R)fa.bfx, Fy = (R)fa.bfy, Fz = (R)fa.bfz;
     if (fa.gcur) {
       R gx, gy, gz;
-      if constexpr (sizeof(R) == 4)
-        force_row<R>(fa.gcur32 + 18ull * row, c1, s1, c2, s2, gx, gy, gz);
-      else
-        force_row<R>(fa.gcur + 18ull * row, c1, s1, c2, s2, gx, gy, gz);
+      force_row<R>(fa.gcur + 18ull * row, c1, s1, c2, s2, gx, gy, gz);
       Fx += gx;
       Fy += gy;
       Fz += gz;
@@ -285,7 +274,7 @@ __device__ __forceinline__ void pick5(const double (&r)[5], const double (&i)[5]
 
 __global__ void __launch_bounds__(256)
     k_force_table(ModeSet M, const double* __restrict__ phy, const double* __restrict__ phz, int ny,
-                  int nz, double* __restrict__ G, float* __restrict__ G32) {
+                  int nz, double* __restrict__ G) {
   const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= (uint32_t)ny * (uint32_t)nz) return;
   const int y = (int)(row % (uint32_t)ny), z = (int)(row / (uint32_t)ny);
@@ -319,10 +308,6 @@ __global__ void __launch_bounds__(256)
   }
 #pragma unroll
   for (int q = 0; q < 18; ++q) G[18ull * row + q] = g[q];
-  if (G32) {
-#pragma unroll
-    for (int q = 0; q < 18; ++q) G32[18ull * row + q] = (float)g[q];
-  }
 }
 
 // macrovar (collision.f90:232-332): rho = sum f, u = sum c f + F/2, then the
@@ -493,8 +478,6 @@ static ForceArgs force_args(const Ctx& c) {
   ForceArgs fa;
   fa.gcur = c.ft.live[c.ft.cur] ? c.ft.g[c.ft.cur] : nullptr;
   fa.gmac = c.ft.live[c.ft.mac] ? c.ft.g[c.ft.mac] : nullptr;
-  fa.gcur32 = c.ft.live[c.ft.cur] ? c.ft.g32[c.ft.cur] : nullptr;
-  fa.gmac32 = c.ft.live[c.ft.mac] ? c.ft.g32[c.ft.mac] : nullptr;
   fa.phx = c.ft.phx;
   fa.bfx = c.p.body_force[0];
   fa.bfy = c.p.body_force[1];
@@ -505,7 +488,7 @@ static ForceArgs force_args(const Ctx& c) {
 hipError_t launch_force_table(Ctx& c, int slot, const ModeSet& m) {
   const uint32_t rows = (uint32_t)c.g.ny * (uint32_t)c.g.nz;
   hipLaunchKernelGGL(k_force_table, dim3((rows + 255) / 256), dim3(256), 0, c.stream, m,
-                     c.ft.phy, c.ft.phz, c.g.ny, c.g.nz, c.ft.g[slot], c.ft.g32[slot]);
+                     c.ft.phy, c.ft.phz, c.g.ny, c.g.nz, c.ft.g[slot]);
   return hipGetLastError();
 }
 
@@ -527,7 +510,6 @@ CollideArgs collide_args(Ctx& c) {
   a.mask = (c.pt.np > 0 && c.pt.map_np[c.pt.map_cur] > 0) ? c.pt.mask[c.pt.map_cur] : nullptr;
   a.g = c.g;
   a.k = c.cc;
-  a.kf = to_f32(c.cc);
   a.fa = force_args(c);
   a.rho = c.rho; a.ux = c.ux; a.uy = c.uy; a.uz = c.uz;
   a.arr = 0;
@@ -689,7 +671,6 @@ hipError_t launch_aos_to_pop(Ctx& c, const double* src, int z0, int nzs, bool pu
 hipError_t launch_force_field(Ctx& c, int slot, double* fx, double* fy, double* fz) {
   ForceArgs fa = force_args(c);
   fa.gcur = c.ft.live[slot] ? c.ft.g[slot] : nullptr;
-  fa.gcur32 = c.ft.live[slot] ? c.ft.g32[slot] : nullptr;
   hipLaunchKernelGGL(k_force_field, dim3(nblocks(c.g.ncell)), dim3(256), 0, c.stream, c.g, fa, fx, fy, fz);
   return hipGetLastError();
 }

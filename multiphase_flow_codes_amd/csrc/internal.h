@@ -55,7 +55,6 @@ struct ModeSet {
 
 struct ForceTables {
   double* g[kForceSlots] = {};                 // per (y,z) row: 3 comps x 3 kx x (re,im)
-  float* g32[kForceSlots] = {};                // the same rounded to FP32 (FP32 contexts)
   int live[kForceSlots] = {};                  // 1: table holds nonzero modes
   int cur = 0;                                 // modes of the last FORCING
   int mac = 1;                                 // modes captured by the last macrovar

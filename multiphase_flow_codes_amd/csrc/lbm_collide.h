@@ -88,10 +88,9 @@ MFC_HD void forcing_term(R (&Fb)[NPOP], R ux, R uy, R uz, R Fx, R Fy, R Fz, R w1
 // conserved corrections are exactly dR = 0 and dJ = (F - Fm)/20, since
 // sum Fbar = 0 and sum c Fbar = F; taking them in closed form avoids the
 // cancellation of two rounded 19-term sums.
-// K: CollideConsts, or CollideConstsF for FP32 arithmetic.
-template <typename R, bool EXACT_CONS, typename K>
+template <typename R, bool EXACT_CONS>
 MFC_HD void mrt_corrections(const ClassSums<R>& s, R rho, R ux, R uy, R uz, R Fx, R Fy, R Fz,
-                            const K& k, R mFx, R mFy, R mFz, R (&corr)[NPOP]) {
+                            const CollideConsts& k, R mFx, R mFy, R mFz, R (&corr)[NPOP]) {
   const R sa = s.sx + s.sy + s.sz;
   const R exy = s.xy_e + s.xy_o, exz = s.xz_e + s.xz_o, eyz = s.yz_e + s.yz_o;
   const R se = exy + exz + eyz;
@@ -193,16 +192,16 @@ MFC_HD void mrt_corrections(const ClassSums<R>& s, R rho, R ux, R uy, R uz, R Fx
 //            post-collision populations
 //   F*       the forcing field at this node for the current step
 // (rho, u*, EXACT_CONS and mF* as for mrt_corrections)
-template <typename R, bool EXACT_CONS = false, typename K = CollideConsts>
+template <typename R, bool EXACT_CONS = false>
 MFC_HD void mrt_collide(R (&f)[NPOP], R rho, R ux, R uy, R uz, R Fx, R Fy, R Fz,
-                        const K& k, R mFx = 0, R mFy = 0, R mFz = 0) {
+                        const CollideConsts& k, R mFx = 0, R mFy = 0, R mFz = 0) {
   R Fb[NPOP];
   forcing_term<R>(Fb, ux, uy, uz, Fx, Fy, Fz, (R)k.ww1, (R)k.ww2);
   R g[NPOP];
 #pragma unroll
   for (int i = 0; i < NPOP; ++i) g[i] = f[i] + (R)0.5 * Fb[i];
   R corr[NPOP];
-  mrt_corrections<R, EXACT_CONS, K>(class_sums(g), rho, ux, uy, uz, Fx, Fy, Fz, k, mFx, mFy, mFz, corr);
+  mrt_corrections<R, EXACT_CONS>(class_sums(g), rho, ux, uy, uz, Fx, Fy, Fz, k, mFx, mFy, mFz, corr);
 #pragma unroll
   for (int i = 0; i < NPOP; ++i) f[i] = (g[i] - corr[i]) + (R)0.5 * Fb[i];
 }

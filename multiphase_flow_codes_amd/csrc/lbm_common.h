@@ -53,23 +53,6 @@ struct CollideConsts {
   double bfx, bfy, bfz;                   // uniform body force added to the forcing field
 };
 
-// The same constants rounded to FP32 once on the host: FP32 kernels read
-// these instead of converting the FP64 ones in every wavefront (the values
-// are identical to a conversion at the point of use).
-struct CollideConstsF {
-  float s1, s2, s4, s9, s10, s13, s16;
-  float omegepsl, omegepslj, omegxx;
-  float ww0, ww1, ww2;
-  float bfx, bfy, bfz;
-};
-
-MFC_HD CollideConstsF to_f32(const CollideConsts& k) {
-  return CollideConstsF{(float)k.s1, (float)k.s2, (float)k.s4, (float)k.s9, (float)k.s10,
-                        (float)k.s13, (float)k.s16, (float)k.omegepsl, (float)k.omegepslj,
-                        (float)k.omegxx, (float)k.ww0, (float)k.ww1, (float)k.ww2,
-                        (float)k.bfx, (float)k.bfy, (float)k.bfz};
-}
-
 // Population layouts of the resident buffer (where the post-collision value
 // of direction i at node n lives).
 //   LAY_AB       two buffers: the current one, plane i, slot n

@@ -9,8 +9,6 @@ namespace mfc {
 struct ForceArgs {
   const double* gcur;   // forcing table of the current step (may be null)
   const double* gmac;   // forcing table captured by the last macrovar (may be null)
-  const float* gcur32;  // FP32 copies of the two (FP32 contexts; null otherwise)
-  const float* gmac32;
   const double* phx;    // (cos,sin)(2 pi m / nx)
   double bfx, bfy, bfz; // uniform body force
 };
@@ -22,7 +20,6 @@ struct CollideArgs {
   const uint32_t* mask;
   Grid g;
   CollideConsts k;
-  CollideConstsF kf;   // k rounded to FP32, for FP32 arithmetic
   ForceArgs fa;
   const double *rho, *ux, *uy, *uz;
   int arr;   // batched launches: conserved moments from the arrays (else recomputed)

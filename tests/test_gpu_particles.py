@@ -8,7 +8,7 @@ normwise relative error in populations and particle state.
 import numpy as np
 import pytest
 
-from helpers import fluid_step_both, place_spheres, small_params, start_both
+from helpers import fluid_step_both, place_spheres, small_params, smooth_velocity, start_both
 
 pytestmark = pytest.mark.gpu
 
@@ -155,9 +155,10 @@ def test_moving_spheres_refill_nodes(oracle_lib):
 
 
 def test_multistep_call_equals_fine_grained(oracle_lib):
-    """One mfc_step call over several steps (forcing tables, lubrication and
-    map clearing overlapped on a second stream) gives bitwise the same state
-    as the routine-by-routine sequence."""
+    """One mfc_step call over several steps (lubrication and map clearing in
+    the bounce-back grids, move and refill search fused) gives bitwise the
+    same state as the routine-by-routine sequence, the forcing process state
+    included."""
     from multiphase_flow_codes_amd.lbm import Lattice
     rng = np.random.default_rng(11)
     prm = small_params(n=(32, 32, 32), npart=3, rad=4.0, rhopart=1.5)
@@ -202,6 +203,12 @@ def test_multistep_call_equals_fine_grained(oracle_lib):
         ib, sb = b.get_maps(which)
         assert np.array_equal(ia, ib) and np.array_equal(sa, sb)
     assert a.status()["refill"] == b.status()["refill"]
+    fa, fb = a.get_forcing_state(), b.get_forcing_state()
+    assert (fa["iseedf"], fa["iyf"]) == (fb["iseedf"], fb["iyf"])
+    assert np.array_equal(fa["ivf"], fb["ivf"])
+    for key in ("a", "b"):
+        for x, y in zip(fa[key], fb[key]):
+            assert np.array_equal(x, y), key
 
 
 def test_body_and_sphere_forces_match_oracle(oracle_lib):
@@ -272,3 +279,28 @@ def test_channel_initpartpos_matches_oracle(oracle_lib):
     seed_out = lat.initpartpos(prm.iseedp, mingap_check=True)
     assert np.array_equal(lat.prt_get("ypglb"), orc.ypglb)
     assert seed_out == orc.S.iseedp
+
+
+def test_failed_step_puts_forcing_state_back(oracle_lib):
+    """A step that fails after its FORCING (here: spheres without
+    beads_links) leaves the forcing process as that one call leaves it."""
+    from multiphase_flow_codes_amd.lbm import Lattice
+    prm = small_params(n=(32, 32, 32), npart=2, rad=4.0)
+    rng = np.random.default_rng(2)
+    a, b = Lattice(prm, "fp64"), Lattice(prm, "fp64")
+    for lat in (a, b):
+        lat.upload_macro(None, *smooth_velocity(prm, np.random.default_rng(3)))
+        lat.initpop()
+        lat.rhoupdat()
+        lat.macrovar()
+    a.prt_init(prm.npart)
+    a.prt_set("ypglb", place_spheres(prm, rng, prm.npart, gap=2.5))
+    with pytest.raises(Exception):
+        a.step(1, 3)
+    b.forcing()
+    fa, fb = a.get_forcing_state(), b.get_forcing_state()
+    assert (fa["iseedf"], fa["iyf"]) == (fb["iseedf"], fb["iyf"])
+    assert np.array_equal(fa["ivf"], fb["ivf"])
+    for key in ("a", "b"):
+        for x, y in zip(fa[key], fb[key]):
+            assert np.array_equal(x, y), key
