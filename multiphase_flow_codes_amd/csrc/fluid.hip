@@ -272,10 +272,24 @@ __device__ __forceinline__ void pick5(const double (&r)[5], const double (&i)[5]
   }
 }
 
+// The mode list is first copied into LDS by all threads at once: read from
+// the kernel arguments inside the mode loop it is a chain of dependent
+// scalar loads (each mode's ky, kz select the next phases), which set the
+// kernel's time.
 __global__ void __launch_bounds__(256)
     k_force_table(ModeSet M, const double* __restrict__ phy, const double* __restrict__ phz, int ny,
                   int nz, double* __restrict__ G) {
-  const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double sa[kMaxModes * 6];
+  __shared__ int sk[kMaxModes];   // ky + 8 * kz, both in -2..2
+  const int nm = M.n[0] + M.n[1] + M.n[2];
+  for (int t = threadIdx.x; t < nm * 6; t += blockDim.x) sa[t] = (&M.a[0][0])[t];
+  for (int t = threadIdx.x; t < nm; t += blockDim.x) sk[t] = (int)M.ky[t] + 8 * (int)M.kz[t];
+  __syncthreads();
+  // one thread per (row, component): a row's 18 doubles come from three
+  // adjacent lanes, 6 each, so a wavefront stores one contiguous run
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t row = t / 3u;
+  const int cc = (int)(t - 3u * row);
   if (row >= (uint32_t)ny * (uint32_t)nz) return;
   const int y = (int)(row % (uint32_t)ny), z = (int)(row / (uint32_t)ny);
   double eyr[5], eyi[5], ezr[5], ezi[5];
@@ -287,27 +301,28 @@ __global__ void __launch_bounds__(256)
     ezr[d + 2] = phz[2 * mz];
     ezi[d + 2] = phz[2 * mz + 1];
   }
-  double g[18];
+  double g[6];
 #pragma unroll
-  for (int q = 0; q < 18; ++q) g[q] = 0.0;
+  for (int q = 0; q < 6; ++q) g[q] = 0.0;
   int m = 0;
 #pragma unroll
   for (int kx = 0; kx < 3; ++kx) {
     for (int e = 0; e < M.n[kx]; ++e, ++m) {
+      // the same mode for every lane: uniform selection of the phases
+      const int k = __builtin_amdgcn_readfirstlane(sk[m]);
+      const int kz = (k + 2 * 8 + 2) / 8 - 2, ky = k - 8 * kz;
       double a, b, c, d;
-      pick5(eyr, eyi, M.ky[m], a, b);
-      pick5(ezr, ezi, M.kz[m], c, d);
+      pick5(eyr, eyi, ky, a, b);
+      pick5(ezr, ezi, kz, c, d);
       const double er = a * c - b * d, ei = a * d + b * c;
-#pragma unroll
-      for (int cc = 0; cc < 3; ++cc) {
-        const double ar = M.a[m][2 * cc], ai = M.a[m][2 * cc + 1];
-        g[6 * cc + 2 * kx] += ar * er - ai * ei;
-        g[6 * cc + 2 * kx + 1] += ar * ei + ai * er;
-      }
+      const double ar = sa[6 * m + 2 * cc], ai = sa[6 * m + 2 * cc + 1];
+      g[2 * kx] += ar * er - ai * ei;
+      g[2 * kx + 1] += ar * ei + ai * er;
     }
   }
+  double* out = G + 18ull * row + 6 * cc;
 #pragma unroll
-  for (int q = 0; q < 18; ++q) G[18ull * row + q] = g[q];
+  for (int q = 0; q < 6; ++q) out[q] = g[q];
 }
 
 // macrovar (collision.f90:232-332): rho = sum f, u = sum c f + F/2, then the
@@ -487,7 +502,7 @@ static ForceArgs force_args(const Ctx& c) {
 
 hipError_t launch_force_table(Ctx& c, int slot, const ModeSet& m) {
   const uint32_t rows = (uint32_t)c.g.ny * (uint32_t)c.g.nz;
-  hipLaunchKernelGGL(k_force_table, dim3((rows + 255) / 256), dim3(256), 0, c.stream, m,
+  hipLaunchKernelGGL(k_force_table, dim3((3 * rows + 255) / 256), dim3(256), 0, c.stream, m,
                      c.ft.phy, c.ft.phz, c.g.ny, c.g.nz, c.ft.g[slot]);
   return hipGetLastError();
 }
