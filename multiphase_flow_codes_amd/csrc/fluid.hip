@@ -99,12 +99,12 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
   const uint32_t xs[3] = {xpp, (uint32_t)x, xm};   // cx = -1, 0, +1
   const uint32_t n = (uint32_t)x + nx * row;
   const bool wall_lo = WALLS && y == 0, wall_hi = WALLS && y == g.ny - 1;
-  // location of the streamed value of direction i: plane offset (pl * ncell,
+  // location of the streamed value of direction i: plane offset (pl * pitch,
   // a select between two compile-time planes on the wave-uniform wall test,
   // so it stays a scalar) and slot
   auto src_of = [&](int i, uint64_t& pbase, uint32_t& src) {
     if (LAY == LAY_AA_ODD) {
-      pbase = (uint64_t)i * g.ncell;
+      pbase = (uint64_t)i * g.pitch;
       src = n;
       return;
     }
@@ -113,7 +113,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
     src = wall ? n : xs[ccx + 1] + nx * (yy[ccy + 1] + ny * zz[ccz + 1]);
     const int p_in = LAY == LAY_AB ? i : opp(i);       // plane of an interior pull
     const int p_wall = LAY == LAY_AB ? opp(i) : i;     // plane beyond a wall
-    const uint64_t b_in = (uint64_t)p_in * g.ncell, b_wall = (uint64_t)p_wall * g.ncell;
+    const uint64_t b_in = (uint64_t)p_in * g.pitch, b_wall = (uint64_t)p_wall * g.pitch;
     pbase = wall ? b_wall : b_in;
   };
   R f[NPOP];
@@ -172,7 +172,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
 #pragma unroll
     for (int i = 0; i < NPOP; ++i) {
       __builtin_nontemporal_store((T)f[i], q);
-      q += g.ncell;
+      q += g.pitch;
     }
   } else {
 #pragma unroll

@@ -87,7 +87,7 @@ __device__ __forceinline__ void load_pulled(const T* A, const Grid& g, int x, in
     int pl;
     uint32_t s;
     pull_src<WALLS>(i, x, y, z, n, g, pl, s);
-    f[i] = (R)A[(uint64_t)pl * g.ncell + s];
+    f[i] = (R)A[(uint64_t)pl * g.pitch + s];
   }
 }
 
@@ -101,7 +101,7 @@ __device__ __forceinline__ void load_direct(const T* A, const Grid& g, int x, in
     int pl;
     uint32_t s;
     post_src<WALLS>(g.lay, i, x, y, z, n, g, pl, s);
-    f[i] = (R)A[(uint64_t)pl * g.ncell + s];
+    f[i] = (R)A[(uint64_t)pl * g.pitch + s];
   }
 }
 
@@ -117,13 +117,13 @@ __device__ __forceinline__ void store_phase(T* A, const Grid& g, int i, int x, i
     pull_src<WALLS>(i, x, y, z, n, g, pl, s);
   else
     post_src<WALLS>(g.lay, i, x, y, z, n, g, pl, s);
-  A[(uint64_t)pl * g.ncell + s] = (T)v;
+  A[(uint64_t)pl * g.pitch + s] = (T)v;
 }
 
 // Value of direction i at node n after collision, read by the bounce-back.
 template <typename T>
 __device__ __forceinline__ double post_value(const T* A, const Grid& g, int i, uint32_t n) {
-  if (g.lay == LAY_AB) return (double)A[(uint64_t)i * g.ncell + n];
+  if (g.lay == LAY_AB) return (double)A[(uint64_t)i * g.pitch + n];
   int x, y, z;
   uint32_t row;
   node_coords(n, g, x, y, z, row);
@@ -133,7 +133,7 @@ __device__ __forceinline__ double post_value(const T* A, const Grid& g, int i, u
     post_src<true>(g.lay, i, x, y, z, n, g, pl, s);
   else
     post_src<false>(g.lay, i, x, y, z, n, g, pl, s);
-  return (double)A[(uint64_t)pl * g.ncell + s];
+  return (double)A[(uint64_t)pl * g.pitch + s];
 }
 
 }  // namespace mfc

@@ -70,6 +70,7 @@ struct Grid {
   int walls_y;        // 1: half-way bounce-back walls on the y faces
   int lay;            // PopLayout of the resident buffer
   uint64_t ncell;     // nx*ny*nz
+  uint64_t pitch;     // elements from one population plane to the next (>= ncell)
 };
 
 MFC_HD int wrapi(int a, int n) {

@@ -900,7 +900,7 @@ __device__ __forceinline__ void bounce_back_body(int b_, const BounceArgs& a) {
         post_src<true>(g.lay, opp(q), xt, ytw, zt, t, g, pl, s);
       else
         post_src<false>(g.lay, opp(q), xt, ytw, zt, t, g, pl, s);
-      A[(uint64_t)pl * g.ncell + s] = (T)f9;
+      A[(uint64_t)pl * g.pitch + s] = (T)f9;
     }
     const double dff = ff1 + f9;
     const double dxm = dff * (double)cx(q), dym = dff * (double)cy(q), dzm = dff * (double)cz(q);

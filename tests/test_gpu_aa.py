@@ -154,3 +154,23 @@ def test_aa_refused_by_ensemble():
     finally:
         ens.close()
         lat.close()
+
+
+@pytest.mark.parametrize("storage", ["ab", "aa"])
+def test_padded_plane_pitch_is_bitwise_neutral(storage, monkeypatch):
+    """Population planes spaced wider than the node count (MFC_PLANE_PAD,
+    in 256-B units) change addresses only: fused steps with spheres and
+    forcing give bitwise the same field and sphere state."""
+    name = "periodic"
+    prm = small_params(**CASES[name])
+    pos = _positions(name, prm, np.random.default_rng(3))
+    out = []
+    for pad in ("0", "17"):
+        monkeypatch.setenv("MFC_PLANE_PAD", pad)
+        lat = _released(prm, storage, "fp64", pos, 7)
+        lat.step(1, 5)
+        lat.sync()
+        out.append((lat.download_f(), lat.prt_get("ypglb"), lat.prt_get("fHIp")))
+        lat.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
