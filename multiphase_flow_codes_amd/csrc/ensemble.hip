@@ -116,6 +116,7 @@ int check_member(mfc_ens* e, Ctx& c) {
 int ens_one_step(mfc_ens* e) {
   StageTab coll, st[7];
   coll.arg_bytes = sizeof(CollideArgs);
+  bool arr_any = false;
   const Stage order[7] = {SG_CONTACT, SG_BOUNCE, SG_MOVE, SG_SCAN, SG_COUNT, SG_EXPAND, SG_FILL};
   for (int k = 0; k < 7; ++k) st[k].arg_bytes = stage_arg_bytes(order[k]);
   for (Member& mb : e->m) {
@@ -130,6 +131,7 @@ int ens_one_step(mfc_ens* e) {
         return efail(e, MFC_ERR_STATE, "ensemble: member moments out of date");
       CollideArgs a = collide_args(c);
       a.arr = c.mom == MOM_ARRAYS ? 1 : 0;
+      arr_any = arr_any || a.arr;
       coll.add(&a, (int)((c.g.ncell + e->collide_bs - 1) / e->collide_bs));
       c.cur ^= 1;
       c.phase = PHASE_PRE_STREAM;
@@ -204,7 +206,7 @@ int ens_one_step(mfc_ens* e) {
   auto enqueue = [&]() -> int {
     EHIP(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, e->stream));
     EHIP(launch_batched_collide(e->prec, e->walls, e->collide_bs, d + pos[0][0], (const int*)(d + pos[0][1]),
-                                coll.nc, coll.off.back(), e->stream));
+                                coll.nc, coll.off.back(), arr_any, e->stream));
     for (int k = 0; k < 7; ++k) {
       StageTab& t = st[k];
       EHIP(launch_batched(order[k], e->prec, e->walls, d + pos[k + 1][0], (const int*)(d + pos[k + 1][1]),
@@ -219,9 +221,10 @@ int ens_one_step(mfc_ens* e) {
     return MFC_OK;
   }
   // what the captured work depends on besides the table contents: buffers,
-  // sizes, launch shapes and the collide block size
+  // sizes, launch shapes and the collide kernel (block size, array moments)
   std::vector<long long> sig = {(long long)(uintptr_t)d, (long long)(uintptr_t)h, (long long)total,
-                                e->collide_bs, e->prec, e->walls, coll.nc, coll.off.back()};
+                                e->collide_bs, e->prec, e->walls, coll.nc, coll.off.back(),
+                                arr_any ? 1LL : 0LL};
   for (int k = 0; k < 7; ++k) {
     sig.push_back(st[k].nc);
     sig.push_back(st[k].off.back());

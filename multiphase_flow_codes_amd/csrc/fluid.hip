@@ -240,7 +240,9 @@ __device__ __forceinline__ int case_of_block(const int* __restrict__ off, int nc
   return lo;
 }
 
-template <typename T, typename R, bool WALLS, int BS>
+// ARR_ANY: some member takes its moments from the rho/u arrays (a.arr, its
+// first step after joining); without it only the recomputing path is built.
+template <typename T, typename R, bool WALLS, int BS, bool ARR_ANY>
 __global__ void __launch_bounds__(BS)
     kb_collide(const CollideArgs* __restrict__ t, const int* __restrict__ off, int nc) {
   const int c = case_of_block(off, nc, (int)blockIdx.x);
@@ -250,7 +252,7 @@ __global__ void __launch_bounds__(BS)
   const uint32_t nx = (uint32_t)a.g.nx, ny = (uint32_t)a.g.ny;
   const uint32_t x = (uint32_t)(n % nx);
   const uint32_t r = (uint32_t)(n / nx);
-  if (a.arr)
+  if (ARR_ANY && a.arr)
     collide_node<T, R, true, WALLS, LAY_AB>(a, (int)x, (int)(r % ny), (int)(r / ny));
   else
     collide_node<T, R, false, WALLS, LAY_AB>(a, (int)x, (int)(r % ny), (int)(r / ny));
@@ -694,35 +696,43 @@ hipError_t launch_force_field(Ctx& c, int slot, double* fx, double* fy, double* 
 
 namespace mfc {
 
-template <int BS>
+template <typename T, typename R, int BS, bool ARR_ANY>
+static void batched_collide_k(int walls, const CollideArgs* tab, const int* off, int nc, int blocks,
+                              hipStream_t s) {
+  if (walls)
+    hipLaunchKernelGGL((kb_collide<T, R, true, BS, ARR_ANY>), dim3(blocks), dim3(BS), 0, s, tab, off, nc);
+  else
+    hipLaunchKernelGGL((kb_collide<T, R, false, BS, ARR_ANY>), dim3(blocks), dim3(BS), 0, s, tab, off, nc);
+}
+
+template <int BS, bool ARR_ANY>
 static void batched_collide_bs(int prec, int walls, const void* t, const int* off, int nc, int blocks,
                                hipStream_t s) {
   const CollideArgs* tab = (const CollideArgs*)t;
-  if (prec == MFC_PREC_FP64) {
-    if (walls)
-      hipLaunchKernelGGL((kb_collide<double, double, true, BS>), dim3(blocks), dim3(BS), 0, s, tab, off, nc);
-    else
-      hipLaunchKernelGGL((kb_collide<double, double, false, BS>), dim3(blocks), dim3(BS), 0, s, tab, off, nc);
-  } else if (prec == MFC_PREC_FP32_F64MATH) {
-    if (walls)
-      hipLaunchKernelGGL((kb_collide<float, double, true, BS>), dim3(blocks), dim3(BS), 0, s, tab, off, nc);
-    else
-      hipLaunchKernelGGL((kb_collide<float, double, false, BS>), dim3(blocks), dim3(BS), 0, s, tab, off, nc);
-  } else {
-    if (walls)
-      hipLaunchKernelGGL((kb_collide<float, float, true, BS>), dim3(blocks), dim3(BS), 0, s, tab, off, nc);
-    else
-      hipLaunchKernelGGL((kb_collide<float, float, false, BS>), dim3(blocks), dim3(BS), 0, s, tab, off, nc);
-  }
+  if (prec == MFC_PREC_FP64)
+    batched_collide_k<double, double, BS, ARR_ANY>(walls, tab, off, nc, blocks, s);
+  else if (prec == MFC_PREC_FP32_F64MATH)
+    batched_collide_k<float, double, BS, ARR_ANY>(walls, tab, off, nc, blocks, s);
+  else
+    batched_collide_k<float, float, BS, ARR_ANY>(walls, tab, off, nc, blocks, s);
 }
 
 hipError_t launch_batched_collide(int prec, int walls, int bs, const void* t, const int* off, int nc,
-                                  int blocks, hipStream_t s) {
+                                  int blocks, bool arr_any, hipStream_t s) {
   if (nc <= 0 || blocks <= 0) return hipSuccess;
   switch (bs) {
-    case 64: batched_collide_bs<64>(prec, walls, t, off, nc, blocks, s); break;
-    case 128: batched_collide_bs<128>(prec, walls, t, off, nc, blocks, s); break;
-    case 256: batched_collide_bs<256>(prec, walls, t, off, nc, blocks, s); break;
+    case 64:
+      arr_any ? batched_collide_bs<64, true>(prec, walls, t, off, nc, blocks, s)
+              : batched_collide_bs<64, false>(prec, walls, t, off, nc, blocks, s);
+      break;
+    case 128:
+      arr_any ? batched_collide_bs<128, true>(prec, walls, t, off, nc, blocks, s)
+              : batched_collide_bs<128, false>(prec, walls, t, off, nc, blocks, s);
+      break;
+    case 256:
+      arr_any ? batched_collide_bs<256, true>(prec, walls, t, off, nc, blocks, s)
+              : batched_collide_bs<256, false>(prec, walls, t, off, nc, blocks, s);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

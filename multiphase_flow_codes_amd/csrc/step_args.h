@@ -198,8 +198,9 @@ bool fill_apply_args(Ctx& c, int nblk, FillApplyArgs& a);
 enum Stage { SG_CONTACT, SG_BOUNCE, SG_MOVE, SG_SCAN, SG_COUNT, SG_EXPAND, SG_FILL };
 hipError_t launch_batched(Stage st, int prec, int walls, const void* t, const int* off, int nc,
                           int blocks, size_t lds, hipStream_t s);
+// arr_any: some member's CollideArgs::arr is set
 hipError_t launch_batched_collide(int prec, int walls, int bs, const void* t, const int* off, int nc,
-                                  int blocks, hipStream_t s);
+                                  int blocks, bool arr_any, hipStream_t s);
 size_t stage_arg_bytes(Stage st);
 
 }  // namespace mfc
