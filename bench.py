@@ -30,8 +30,8 @@ BYTES_PER_CELL = {"fp32": 152, "fp32x": 152, "fp64": 304}   # 19 populations rea
 # WRITE_SIZE on these workloads (FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0, the
 # factors a plain 1 GiB device copy gives): profiles/hbm_traffic_box256_channel1024.md
 # (in-place storage: the mean of its two alternating layouts)
-MEASURED_TRAFFIC = {("box256", "fp32", "ab"): 2.65084e9, ("box256", "fp64", "ab"): 5.20217e9,
-                    ("channel1024", "fp32", "ab"): 1.70761e11, ("channel1024", "fp64", "aa"): 3.29957e11}
+MEASURED_TRAFFIC = {("box256", "fp32", "ab"): 2.65395e9, ("box256", "fp64", "ab"): 5.20451e9,
+                    ("channel1024", "fp32", "ab"): 1.70812e11, ("channel1024", "fp64", "aa"): 3.29995e11}
 
 
 def log(msg):
