@@ -1,7 +1,8 @@
 // fluid.hip -- lattice kernels of libmfc_lbm for gfx950.
 //
-// Storage: populations are SoA planes pop[i][n], n = x + nx*(y + ny*z), in
-// FP64 or FP32 (deviations f_i - w_i*rho0).  The resident buffer holds the
+// Storage: populations are SoA planes pop[i * pitch + n], n = x + nx*(y + ny*z)
+// (pitch = the node count plus 512 B), in FP64 or FP32 (deviations
+// f_i - w_i*rho0).  The resident buffer holds the
 // post-collision (pre-streaming) field; streaming is never a separate pass:
 // every consumer pulls f_i(x) = buf[i][x - c_i] (half-way bounce-back at the
 // y walls when they are on).  One kernel therefore streams, takes moments,
