@@ -375,8 +375,8 @@ def main():
         hbm_ceilings()
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a "
-                                   "design note (z-slab halos), not an implemented path",
+        print(json.dumps({"error": "the library runs on one GPU; sharding over several GPUs "
+                                   "(z-slab halos) is designed but not implemented",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     from multiphase_flow_codes_amd import workloads

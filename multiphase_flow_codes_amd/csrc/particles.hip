@@ -507,7 +507,7 @@ __device__ __forceinline__ void scan_counts_body(int b_, const int32_t* __restri
                   int32_t* __restrict__ fill_count, int32_t* __restrict__ ct_count) {
   if (threadIdx.x == 0) {
     *fill_count = 0;   // the next beads_filling starts an empty list
-    *ct_count = 0;     // the expansion lists this build's contact links
+    *ct_count = 0;     // the expansion lists this rebuild's contact links
   }
   __shared__ int64_t part[1024];
   __shared__ int64_t spart[1024];
@@ -1295,7 +1295,7 @@ __global__ void __launch_bounds__(PB) k_count_offsets(ScanCountArgs a) {
   const int b = blockIdx.x;
   if (b == 0 && threadIdx.x == 0) {
     *a.fill_count = 0;   // the next beads_filling starts an empty list
-    *a.ct_count = 0;     // the expansion lists this build's contact links
+    *a.ct_count = 0;     // the expansion lists this rebuild's contact links
   }
   // chunks before this one (and all chunks, for the totals)
   const int nb = gridDim.x;
