@@ -1255,10 +1255,12 @@ __device__ __forceinline__ int case_of(const int* __restrict__ off, int nc, int 
   return lo;
 }
 
-__global__ void __launch_bounds__(PB) k_links_scan(ScanArgs a) {
+// 8 waves per SIMD (the compiler fits the scan in 64 VGPRs without spills):
+// a box of a few hundred spheres then runs its segments in one round
+__global__ void __launch_bounds__(PB) __attribute__((amdgpu_waves_per_eu(8))) k_links_scan(ScanArgs a) {
   links_scan_body((int)blockIdx.x, a.ypglb, a.np, a.nseg, a.seg_cap, a.G, a.isn, a.mask, a.sol_list, a.sol_cnt, a.sh_node, a.sh_hits, a.sh_cnt, a.lk_cnt, a.ovl, a.istep, a.status, a.cells);
 }
-__global__ void __launch_bounds__(PB)
+__global__ void __launch_bounds__(PB) __attribute__((amdgpu_waves_per_eu(8)))
     kb_links_scan(const ScanArgs* __restrict__ t, const int* __restrict__ off, int nc) {
   const int c = case_of(off, nc, (int)blockIdx.x);
   const ScanArgs a = t[c];
