@@ -686,7 +686,7 @@ int mfc_prt_init(mfc_ctx* ctx, int32_t npart) {
   if (npart > 0) {
     const int need = 2 * (int)ceil(c.p.rad + 2.0) + 6;
     if (c.g.nx < need || c.g.nz < need || (!c.g.walls_y && c.g.ny < need))
-      return fail(ctx, MFC_ERR_ARG, "box too small for the particle radius (need > 2*(rad+2)+5 nodes)");
+      return fail(ctx, MFC_ERR_ARG, "box too small for the particle radius (need 2*ceil(rad+2)+6 nodes along x, z and periodic y)");
     if (!(c.p.rad > 0.0)) return fail(ctx, MFC_ERR_ARG, "particle radius must be positive");
     if ((size_t)segment_capacity(c.p.rad) * sizeof(uint32_t) > 65536)
       return fail(ctx, MFC_ERR_ARG, "particle radius too large for the box-scan kernels (rad <= 27)");
