@@ -9,7 +9,9 @@ bit, see test_oracle_ref.py):
   orc_streaming      numpy.roll (periodic), brute-force wall rule
   orc_collision_mrt  mass/momentum balance; BGK limit (MRT type 2) closed form
   orc_initpop/feqpnt equilibrium moments (density, momentum, momentum flux)
-  orc_beads_links    brute-force solid set and link set; wall point on the sphere
+  orc_beads_links    brute-force solid set and link set; wall point on the sphere;
+                     at 48^3/64^3 it also checks helpers.sphere_counts, the
+                     vectorised counter of the full-size GPU test
   orc_beads_collision  rest/uniform-density force zero; moving-sphere force closed form
   orc_beads_move     constant force/torque, body and sphere forces: uniformly
                      accelerated motion, exact
@@ -285,6 +287,23 @@ def test_links_brute_force_and_wall_point(oracle_lib):
         pt -= N * np.round(pt / N)
         assert abs(np.linalg.norm(pt) - prm.rad) < 1e-12
 
+
+
+@pytest.mark.parametrize("n,npart,rad", [(64, 8, 4.0), (48, 12, 3.3)])
+def test_sphere_counts_checker_matches_oracle(oracle_lib, n, npart, rad):
+    """The vectorised counter the full-size GPU test trusts gives the
+    oracle's solid set and link count (seeded placements, spheres across the
+    periodic seams)."""
+    from helpers import sphere_counts
+    prm = LBMParams(nx=n, ny=n, nz=n, visc=0.01, npart=npart, rad=rad)
+    o = _orc(prm)
+    o.S.istep00 = 0
+    o.initpartpos()
+    o.beads_links()
+    ns, nl, idx = sphere_counts(o.ypglb.copy(), rad, n)
+    solid = np.flatnonzero(o.ibnodes.reshape(-1) > 0)
+    assert ns == len(solid) and np.array_equal(np.unique(idx), solid)
+    assert nl == int((o.ilink > 0).sum())
 
 def _bb_setup(prm, centre, f_node, wp=(0, 0, 0)):
     o = _orc(prm)
