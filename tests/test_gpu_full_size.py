@@ -1,4 +1,5 @@
-"""The 256^3 periodic box with 300 spheres r = 8 at full size.
+"""The 256^3 periodic box with 300 spheres r = 8, and the 1024^3 channel with
+20,000 spheres r = 6, at full size.
 
 At this size the oracle is too slow to step, so the tests check properties
 that hold at any size:
@@ -15,6 +16,8 @@ that hold at any size:
   difference, so the counts are exact (`helpers.sphere_counts`, itself
   checked against the oracle's beads_links on the CPU). The test also checks
   that a second link rebuild from the same positions gives the same maps.
+  In the 1024^3 channel (2^30 nodes, FP32, two buffers of 81.6 GB each) the
+  same comparison runs after the release and after three steps.
 """
 import numpy as np
 import pytest
@@ -102,4 +105,26 @@ def test_maps_and_link_counts_against_brute_force():
     assert (st2["nsolid"], st2["nlinks"]) == (nsolid, nlinks)
     ib2, isn2 = lat.get_maps(0)
     assert np.array_equal(ib, ib2) and np.array_equal(isn, isn2)
+    lat.close()
+
+
+def test_channel1024_maps_and_link_counts_against_brute_force():
+    from multiphase_flow_codes_amd.workloads import CHANNEL1024, setup_lattice
+    case = CHANNEL1024
+    lat, _, _ = setup_lattice(case, "fp32")
+    for it0, k in ((1, 0), (1, 3)):
+        if k:
+            lat.step(it0, k)
+            lat.sync()
+        st = lat.status()
+        _no_faults(st)
+        pos = lat.prt_get("ypglb")[: case.npart]
+        nsolid, nlinks, idx = sphere_counts(pos, case.rad, case.n)
+        assert (st["nsolid"], st["nlinks"]) == (nsolid, nlinks), k
+        ib, isn = lat.get_maps(0)
+        del isn
+        solid = np.flatnonzero(ib.reshape(-1) > 0)
+        del ib
+        assert len(idx) == len(solid)
+        assert np.array_equal(solid, np.unique(idx)), k
     lat.close()
