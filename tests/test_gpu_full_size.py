@@ -18,6 +18,10 @@ that hold at any size:
   that a second link rebuild from the same positions gives the same maps.
   In the 1024^3 channel (2^30 nodes, FP32, two buffers of 81.6 GB each) the
   same comparison runs after the release and after three steps.
+* The 96-box mix (edges 32-128, 1-50 spheres each, 52.8 M nodes) in one
+  FP32 ensemble, stepped as recorded HIP graphs after the block autotuner.
+  The largest box, the box with the most spheres and the smallest box must
+  equal bitwise the same boxes stepped alone.
 """
 import numpy as np
 import pytest
@@ -128,3 +132,37 @@ def test_channel1024_maps_and_link_counts_against_brute_force():
         assert len(idx) == len(solid)
         assert np.array_equal(solid, np.unique(idx)), k
     lat.close()
+
+
+def test_96_box_ensemble_members_equal_boxes_alone():
+    from multiphase_flow_codes_amd import serving
+    from multiphase_flow_codes_amd.lbm import Ensemble
+    cases = serving.draw_cases(96, np.random.default_rng(2026))
+    lats = [serving.make_lattice(c, "fp32") for c in cases]
+    ens = Ensemble()
+    ens.set_graph(True)
+    for lat in lats:
+        ens.add(lat, 1)
+    ens.autotune(3)                   # 3 x 3 steps
+    ens.step(21)
+    ens.sync()
+    nsteps = 30
+    picks = sorted({int(np.argmax([c.cells for c in cases])),
+                    int(np.argmax([c.npart for c in cases])),
+                    int(np.argmin([c.cells for c in cases]))})
+    for i in picks:
+        alone = serving.make_lattice(cases[i], "fp32")
+        alone.step(1, nsteps)
+        alone.sync()
+        member = lats[i]
+        assert np.array_equal(alone.download_f(), member.download_f()), i
+        for name in ("ypglb", "wp", "omgp", "fHIp", "torqp", "flubp"):
+            assert np.array_equal(alone.prt_get(name), member.prt_get(name)), (i, name)
+        for a, b in zip(alone.get_maps(0), member.get_maps(0)):
+            assert np.array_equal(a, b), i
+        alone.close()
+    for lat in lats:
+        ens.remove(lat)
+    ens.close()
+    for lat in lats:
+        lat.close()
