@@ -88,7 +88,7 @@ def test_prerelax_helper_converges_like_the_loop(oracle_lib):
     assert rel(lat.download_f(), orc.f) < 1e-11
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp32x"])
 def test_fused_step_equals_fine_grained(oracle_lib, precision):
     """mfc_step, split into two calls, equals FORCING, collision, streaming,
     macrovar called one by one: populations, forcing field and forcing
@@ -134,12 +134,15 @@ def test_upload_download_roundtrip_both_phases(oracle_lib):
     assert np.array_equal(lat.download_f(), g)
 
 
-def test_fp32_tracks_fp64(oracle_lib):
+@pytest.mark.parametrize("precision", ["fp32", "fp32x"])
+def test_fp32_tracks_fp64(oracle_lib, precision):
+    """FP32 storage (FP32 or FP64 collision arithmetic) stays within 1e-5
+    velocity relative L2 of the FP64 path over 200 forced steps."""
     from multiphase_flow_codes_amd.lbm import Lattice
     prm = small_params(n=(32, 32, 32), strong_forcing=False)
     rng = np.random.default_rng(2)
     ux, uy, uz = smooth_velocity(prm, rng, amp=0.03)
-    lats = {p: Lattice(prm, p) for p in ("fp64", "fp32")}
+    lats = {p: Lattice(prm, p) for p in ("fp64", precision)}
     for lat in lats.values():
         lat.upload_macro(None, ux, uy, uz)
         lat.initpop()
@@ -147,7 +150,7 @@ def test_fp32_tracks_fp64(oracle_lib):
         lat.macrovar()
         lat.step(1, 200)
     u64 = np.stack(lats["fp64"].download_macro()[1:])
-    u32 = np.stack(lats["fp32"].download_macro()[1:])
+    u32 = np.stack(lats[precision].download_macro()[1:])
     assert np.all(np.isfinite(u64))
     err = np.linalg.norm(u32 - u64) / np.linalg.norm(u64)
     assert err < 1e-5, err

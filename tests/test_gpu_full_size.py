@@ -4,9 +4,10 @@
 At this size the oracle is too slow to step, so the tests check properties
 that hold at any size:
 
-* FP32 against the plain FP64 path, 1000 steps from the same state.
-  Velocity relative L2 must be <= 1e-5 and per-particle drag (fHIp)
-  relative L2 <= 1e-4: the accuracy bar of the FP32 path.
+* FP32 storage (with FP32 and with FP64 collision arithmetic) against the
+  plain FP64 path, 1000 steps from the same state. Velocity relative L2 must
+  be <= 1e-5 and per-particle drag (fHIp) relative L2 <= 1e-4: the accuracy
+  bar of the FP32 paths.
 * In-place storage against two buffers in FP64, after odd and even step
   counts: populations, particle state, node maps and link counts bitwise equal.
 * Node maps and link counts against a numpy brute force. It applies the
@@ -47,7 +48,7 @@ def test_fp32_accuracy_against_fp64_after_1000_steps():
     from multiphase_flow_codes_amd.workloads import setup_lattice
     case = _case()
     out = {}
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "fp32x"):
         lat, _, _ = setup_lattice(case, prec)
         lat.step(1, ACC_STEPS)
         lat.sync()
@@ -55,12 +56,14 @@ def test_fp32_accuracy_against_fp64_after_1000_steps():
         _, ux, uy, uz = lat.download_macro()
         out[prec] = (np.stack([ux, uy, uz]), lat.prt_get("fHIp")[: case.npart].copy())
         lat.close()
-    (u32, f32), (u64, f64) = out["fp32"], out["fp64"]
-    du = np.linalg.norm(u32 - u64) / np.linalg.norm(u64)
-    df = np.linalg.norm(f32 - f64) / np.linalg.norm(f64)
-    print(f"velocity rel L2 {du:.3e}, drag rel L2 {df:.3e}")
-    assert du <= 1e-5
-    assert df <= 1e-4
+    u64, f64 = out["fp64"]
+    for prec in ("fp32", "fp32x"):
+        u32, f32 = out[prec]
+        du = np.linalg.norm(u32 - u64) / np.linalg.norm(u64)
+        df = np.linalg.norm(f32 - f64) / np.linalg.norm(f64)
+        print(f"{prec}: velocity rel L2 {du:.3e}, drag rel L2 {df:.3e}")
+        assert du <= 1e-5, prec
+        assert df <= 1e-4, prec
 
 
 def test_in_place_storage_bitwise_equal_at_full_size():

@@ -7,7 +7,7 @@ of a bounce-back segment over several workgroups (MFC_BB_SPLIT) only change
 which workgroup computes which nodes or links. Every setting must match the
 oracle to 1e-12 in FP64: the split changes the order of the force sums, so
 1e-12 rather than bitwise. The tile orders must also agree bitwise with each
-other, in FP32 and in FP64 in-place storage.
+other, in FP32 (both arithmetic forms) and in FP64 in-place storage.
 
 Box 544 x 18 x 19: three x-chunks per row (256, 256 and a ragged 32), and
 342 rows, so both remaps also have tiles left over after their last whole
@@ -113,8 +113,9 @@ def _run_remap(monkeypatch, remap, precision, storage, walls):
 @pytest.mark.parametrize("precision,storage,walls", [
     ("fp32", "ab", False),
     ("fp32", "ab", True),
+    ("fp32x", "ab", False),
     ("fp64", "aa", True),
-], ids=["fp32-periodic", "fp32-channel", "fp64-aa-channel"])
+], ids=["fp32-periodic", "fp32-channel", "fp32x-periodic", "fp64-aa-channel"])
 def test_tile_orders_bitwise_equal(monkeypatch, precision, storage, walls):
     runs = [_run_remap(monkeypatch, r, precision, storage, walls) for r in ("0", "1", "2")]
     for k, other in enumerate(runs[1:], start=1):
