@@ -129,11 +129,12 @@ def test_aa_fp64_matches_oracle_with_walls(oracle_lib):
         assert rel(lat.prt_get(name), getattr(orc, name)) < TOL64, name
 
 
-def test_aa_fp32_close_to_two_buffers():
+@pytest.mark.parametrize("precision", ["fp32", "fp32x"])
+def test_aa_fp32_close_to_two_buffers(precision):
     prm = small_params(**CASES["periodic"])
     pos = _positions("periodic", prm, np.random.default_rng(3))
-    ab = _released(prm, "ab", "fp32", pos, 9)
-    aa = _released(prm, "aa", "fp32", pos, 9)
+    ab = _released(prm, "ab", precision, pos, 9)
+    aa = _released(prm, "aa", precision, pos, 9)
     ab.step(1, 9)
     aa.step(1, 9)
     ab.sync()
