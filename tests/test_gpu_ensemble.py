@@ -52,7 +52,8 @@ def _same(a, b):
             assert np.array_equal(ia, ib) and np.array_equal(sa, sb)
 
 
-@pytest.mark.parametrize("precision,graph", [("fp64", False), ("fp32", False), ("fp32", True)])
+@pytest.mark.parametrize("precision,graph", [("fp64", False), ("fp32", False), ("fp32", True),
+                                             ("fp32x", True)])
 def test_ensemble_equals_boxes_alone(precision, graph):
     from multiphase_flow_codes_amd.lbm import Ensemble
     alone = [_box(c, precision) for c in CASES]
