@@ -53,7 +53,7 @@ extern "C" {
  * (BOX/para.f90:1-288, BOX/var_inc.f90:14-29). */
 typedef struct mfc_params {
   int32_t nx, ny, nz;          /* lattice size (var_inc.f90:14) */
-  int32_t precision;           /* MFC_PREC_FP64 (the drop-in) or MFC_PREC_FP32 */
+  int32_t precision;           /* MFC_PREC_FP64 (the drop-in), MFC_PREC_FP32 or MFC_PREC_FP32_F64MATH */
   int32_t npart_max;           /* particle capacity (npart, var_inc.f90:18) */
   int32_t walls_y;             /* 1: half-way bounce-back walls on the y faces */
   int32_t iseedf;              /* forcing RNG state after para (negative: reseed), para.f90:281 */
