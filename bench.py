@@ -337,7 +337,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--case", default="box256", choices=["box64", "box256"])
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--accuracy-steps", type=int, default=1000,
                     help="steps of the FP32-vs-FP64 accuracy comparison (0: skip)")
