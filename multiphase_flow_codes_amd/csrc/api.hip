@@ -19,6 +19,7 @@
 #include <new>
 #include <vector>
 
+#include "force_table.h"
 #include "internal.h"
 
 using namespace mfc;
@@ -161,13 +162,13 @@ void forcing_modes_host(Ctx& c) {
   }
 }
 
-// Build the forcing table of the modes in c.amod into a slot no queued
-// collide or macrovar reads, and make it current.
-int publish_modes(mfc_ctx* ctx) {
+// The table job of the modes in c.amod, into a slot no queued collide or
+// macrovar reads; job.nblk = 0 when no mode is live.
+int prepare_modes(mfc_ctx* ctx, TableJob& job) {
   Ctx& c = ctx->c;
   int slot = 0;
   while (slot == c.ft.cur || slot == c.ft.mac || slot == c.ft.prev) ++slot;
-  ModeSet m;
+  ModeSet& m = job.M;
   int total = 0;
   for (int kx = 0; kx < 3; ++kx) {
     m.n[kx] = 0;
@@ -188,10 +189,32 @@ int publish_modes(mfc_ctx* ctx) {
         ++total;
       }
   }
-  c.ft.live[slot] = total > 0 ? 1 : 0;
-  if (total > 0) HIPCHK(launch_force_table(c, slot, m));
+  const uint32_t rows = (uint32_t)c.g.ny * (uint32_t)c.g.nz;
+  job.phy = c.ft.phy;
+  job.phz = c.ft.phz;
+  job.G = c.ft.g[slot];
+  job.ny = c.g.ny;
+  job.nz = c.g.nz;
+  job.slot = slot;
+  job.nblk = total > 0 ? (int)((3 * rows + 255) / 256) : 0;
+  return MFC_OK;
+}
+
+// The job's table becomes the one the next collide reads.
+void make_current(Ctx& c, const TableJob& job) {
+  c.ft.live[job.slot] = job.nblk > 0 ? 1 : 0;
   c.ft.prev = c.ft.cur;
-  c.ft.cur = slot;
+  c.ft.cur = job.slot;
+}
+
+// Build the forcing table of the modes in c.amod and make it current.
+int publish_modes(mfc_ctx* ctx) {
+  Ctx& c = ctx->c;
+  TableJob job;
+  int rc = prepare_modes(ctx, job);
+  if (rc) return rc;
+  if (job.nblk > 0) HIPCHK(launch_force_table(c, job.slot, job.M));
+  make_current(c, job);
   return MFC_OK;
 }
 
@@ -1024,10 +1047,21 @@ int mfc_step(mfc_ctx* ctx, int32_t istep, int32_t nsteps) {
   Ctx& c = ctx->c;
   int rc;
   Particles& P = c.pt;
+  // FORCING of step s + 1 drawn during step s (spheres on, more steps in
+  // this call): its table is evaluated by extra blocks of step s's refill
+  // grid and made current after step s's macrovar, which keeps step s's
+  // table.  The draw is the same call the next step would make first, so the
+  // forcing state after any return is the one of the routine-by-routine
+  // sequence.
+  bool drawn = false;
+  int drawn_rc = MFC_OK;
+  TableJob job;
   for (int s = 0; s < nsteps; ++s) {
     const int it = istep + s;
     const bool on = particles_on(c) && it >= c.p.irelease;
-    if ((rc = mfc_forcing(ctx))) return rc;
+    if (drawn_rc) return drawn_rc;
+    if (!drawn && (rc = mfc_forcing(ctx))) return rc;
+    drawn = false;
     if ((rc = mfc_collision_mrt(ctx))) return rc;
     if (on) {
       if (P.map_np[P.map_cur] == 0) return fail(ctx, MFC_ERR_STATE, "time step with particles before beads_links");
@@ -1044,10 +1078,16 @@ int mfc_step(mfc_ctx* ctx, int32_t istep, int32_t nsteps) {
       P.map_old = P.map_cur;
       // links with the refill search in the expansion grid, then the refill
       if ((rc = links_now(ctx, 1, it, true))) return rc;
-      HIPCHK(launch_filling(c, false));
+      if (s + 1 < nsteps && c.p.xkf > 0.0) {
+        forcing_modes_host(c);
+        drawn = true;
+        drawn_rc = prepare_modes(ctx, job);
+      }
+      HIPCHK(launch_filling(c, false, drawn && !drawn_rc && job.nblk > 0 ? &job : nullptr));
       c.f_version++;
     }
     if ((rc = mfc_macrovar(ctx))) return rc;
+    if (drawn && !drawn_rc) make_current(c, job);
   }
   return MFC_OK;
 }

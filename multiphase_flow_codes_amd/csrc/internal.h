@@ -187,7 +187,10 @@ hipError_t launch_lubforce(Ctx& c);
 hipError_t launch_map_clear(Ctx& c, int buf);
 hipError_t launch_move(Ctx& c, int first, int wrap, bool from_seg = false);
 hipError_t launch_redistribute(Ctx& c, int istep);
-hipError_t launch_filling(Ctx& c, bool find = true);
+struct TableJob;
+// job: a forcing table evaluated by extra blocks of the refill grid (or by
+// its own launch when there is nothing to refill)
+hipError_t launch_filling(Ctx& c, bool find = true, const TableJob* job = nullptr);
 hipError_t launch_initpartvel(Ctx& c);
 // (re)build the pair-search cells if they are on and stale
 hipError_t ensure_cells(Ctx& c);

@@ -14,6 +14,7 @@
 // Compiled with -ffp-contract=off: the geometry is evaluated exactly as the
 // reference writes it, so maps and wall fractions match the FP64 oracle bit
 // for bit.
+#include "force_table.h"
 #include "internal.h"
 #include "lbm_access.h"
 #include "lbm_particle_math.h"
@@ -1417,9 +1418,14 @@ __global__ void __launch_bounds__(256)
 }
 
 template <typename T, bool WALLS>
-__global__ void __launch_bounds__(256) k_fill_apply(FillApplyArgs a) {
+__global__ void __launch_bounds__(256) k_fill_apply(FillApplyArgs a, TableJob tj) {
+  if ((int)blockIdx.x >= a.nblk) {   // blocks past the refill: the next step's forcing table
+    force_table_block(tj.M, tj.phy, tj.phz, tj.ny, tj.nz, tj.G, blockIdx.x - (uint32_t)a.nblk);
+    return;
+  }
   fill_apply_body<T, WALLS>((int)blockIdx.x, (T*)a.A, a.g, a.fa, a.nblk);
 }
+static_assert(sizeof(FillApplyArgs) + sizeof(TableJob) <= 4096, "refill kernel arguments over 4 KB");
 template <typename T, bool WALLS>
 __global__ void __launch_bounds__(256)
     kb_fill_apply(const FillApplyArgs* __restrict__ t, const int* __restrict__ off, int nc) {
@@ -1755,24 +1761,29 @@ hipError_t launch_redistribute(Ctx& c, int istep) {
 
 constexpr int kFillBlocks = 1024;
 
-hipError_t launch_filling(Ctx& c, bool find) {
+hipError_t launch_filling(Ctx& c, bool find, const TableJob* job) {
   Particles& P = c.pt;
   FillApplyArgs a;
-  if (!fill_apply_args(c, kFillBlocks, a)) return hipSuccess;
+  if (!fill_apply_args(c, kFillBlocks, a)) return job ? launch_force_table(c, job->slot, job->M) : hipSuccess;
   if (find)
     hipLaunchKernelGGL(k_fill_find, dim3(P.map_np[P.map_old] * P.nseg), dim3(PB), 0, c.stream, a.fa,
                        P.nseg, c.d_status);
-  const dim3 grid(kFillBlocks), blk(256);
+  TableJob tj;
+  if (job)
+    tj = *job;
+  else
+    tj.nblk = 0;
+  const dim3 grid(kFillBlocks + tj.nblk), blk(256);
   if (c.prec == MFC_PREC_FP64) {
     if (c.g.walls_y)
-      hipLaunchKernelGGL((k_fill_apply<double, true>), grid, blk, 0, c.stream, a);
+      hipLaunchKernelGGL((k_fill_apply<double, true>), grid, blk, 0, c.stream, a, tj);
     else
-      hipLaunchKernelGGL((k_fill_apply<double, false>), grid, blk, 0, c.stream, a);
+      hipLaunchKernelGGL((k_fill_apply<double, false>), grid, blk, 0, c.stream, a, tj);
   } else {
     if (c.g.walls_y)
-      hipLaunchKernelGGL((k_fill_apply<float, true>), grid, blk, 0, c.stream, a);
+      hipLaunchKernelGGL((k_fill_apply<float, true>), grid, blk, 0, c.stream, a, tj);
     else
-      hipLaunchKernelGGL((k_fill_apply<float, false>), grid, blk, 0, c.stream, a);
+      hipLaunchKernelGGL((k_fill_apply<float, false>), grid, blk, 0, c.stream, a, tj);
   }
   return hipGetLastError();
 }
