@@ -1,17 +1,21 @@
 """Throughput of the D3Q19 MRT step with moving spheres on one MI355X.
 
-Workload (one "step" = one pass of the reference time loop body,
+Headline workload (one "step" = one pass of the reference time-loop body,
 BOX/main.f90:172-188: FORCING, collision_MRT, beads_collision, streaming,
 beads_lubforce, beads_move, beads_redistribute, beads_links, beads_filling,
-macrovar): box256, a 256^3 periodic box, nu = 0.0036, forcing on the
-|k| <= sqrt(2) shell, 300 spheres of radius 8 (density ratio 5), synthetic
-solenoidal initial field (k0 = 4, u_rms = 0.03).  State is resident in HBM
-before the timed region starts.
+macrovar): channel1024, a 1024^3 channel with half-way bounce-back walls on
+the y faces, decaying, nu = 0.002, 20,000 spheres of radius 6 (density ratio
+2.5) at least 8 nodes from the walls, synthetic solenoidal initial field
+(k0 = 8, u_rms = 0.03).  State is resident in HBM before the timed region.
 
-Prints ONE JSON line: MLUPS of the FP32-shifted path (`value`), the plain
-FP64 path beside it, the HBM roofline fraction of the dominant kernel
-(collide-stream, timed with HIP events on its own stream) and the FP64 C
-oracle timed on a bounded sample of the same case on the host cores.
+Prints ONE JSON line: MLUPS of the FP32-shifted path with two population
+buffers (`value`), the plain FP64 path with one buffer updated in place
+beside it (two FP64 buffers do not fit), the HBM roofline fraction of the
+dominant kernel (collide-stream, timed with HIP events on its own stream)
+and of the whole step, FP32-vs-FP64 accuracy after 500 steps with an
+FP64-vs-perturbed-FP64 control, the 256^3 box (config 2) with its own
+accuracy legs, the 96-box ensemble, the serving loop, and the FP64 C oracle
+timed on a bounded sample (a 256^3 section of the channel) on the host cores.
 """
 from __future__ import annotations
 
@@ -28,107 +32,151 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {"fp32": 152, "fp32x": 152, "fp64": 304}   # 19 populations read + 19 written
 # HBM bytes per k_collide dispatch, measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on these workloads (FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0, the
-# factors a plain 1 GiB device copy gives): profiles/hbm_traffic_box256_channel1024.md
-# (in-place storage: the mean of its two alternating layouts)
+# factors a plain 1 GiB device copy gives); in-place storage: the mean of its two
+# alternating layouts
+TRAFFIC_SOURCE = "profiles/hbm_traffic_box256_channel1024.md"
 MEASURED_TRAFFIC = {("box256", "fp32", "ab"): 2.65395e9, ("box256", "fp64", "ab"): 5.20451e9,
                     ("channel1024", "fp32", "ab"): 1.70812e11, ("channel1024", "fp64", "aa"): 3.29995e11}
+KERNEL = {"fp32": "k_collide<float,float>", "fp32x": "k_collide<float,double>",
+          "fp64": "k_collide<double,double>"}
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_gpu(case, precision, steps, warmup, field, case_name, storage="ab", total_steps=0, collect=None):
-    """Set-up, `warmup` untimed steps, `steps` timed steps; then (accuracy
-    legs) untimed steps up to `total_steps` and collect(lat) of the state."""
+def run_gpu(case, precision, steps, warmup, field, case_name, storage="ab", total_steps=0, collect=None,
+            perturb=0.0):
+    """Set-up, `warmup` untimed steps, `steps` timed steps (none when 0);
+    then (accuracy legs) untimed steps up to `total_steps` and collect(lat)
+    of the state."""
     from multiphase_flow_codes_amd.workloads import setup_lattice
     t0 = time.time()
-    lat, prm, field = setup_lattice(case, precision, field, storage=storage)
-    tag = f"{case_name} {precision}{'' if storage == 'ab' else ' ' + storage.upper()}"
+    lat, prm, field = setup_lattice(case, precision, field, storage=storage, perturb=perturb)
+    tag = (f"{case_name} {precision}{'' if storage == 'ab' else ' ' + storage.upper()}"
+           f"{' perturbed' if perturb else ''}")
     log(f"{tag}: set-up {time.time() - t0:.1f} s")
-    lat.step(1, warmup)
-    lat.sync()
-    log(f"{tag}: warm-up {warmup} steps done")
-    # HIP events around a sample of ~25 collide launches of the timed steps
-    # (each event marker idles the GPU for a few microseconds); an odd period,
-    # so in-place storage samples both of its alternating layouts
-    every = max(1, steps // 25)
-    lat.set_timing(every if every % 2 else every + 1)
-    lat.sync()
-    t1 = time.perf_counter()
-    lat.step(1 + warmup, steps)
-    lat.sync()
-    t2 = time.perf_counter()
-    kern_ms, launches = lat.get_timing()
-    lat.set_timing(False)
-    st = lat.status()
-    ms_step = (t2 - t1) * 1e3 / steps
-    ncell = prm.ncell
-    mlups = ncell * steps / (t2 - t1) / 1e6
-    avg_kern = kern_ms / max(launches, 1)
-    achieved = BYTES_PER_CELL[precision] * ncell / (avg_kern * 1e-3) / 1e9
-    log(f"{tag}: {ms_step:.3f} ms/step, {mlups:.0f} MLUPS, collide {avg_kern:.3f} ms "
-        f"({achieved:.0f} GB/s), links {st['nlinks']}, refilled nodes {st['refill']}")
-    state = None
-    if collect is not None:
+    out = {"state": None}
+    done = 0
+    if steps > 0:
+        lat.step(1, warmup)
+        lat.sync()
+        log(f"{tag}: warm-up {warmup} steps done")
+        # HIP events around a sample of ~25 collide launches of the timed steps
+        # (each event marker idles the GPU for a few microseconds); an odd period,
+        # so in-place storage samples both of its alternating layouts
+        every = max(1, steps // 25)
+        lat.set_timing(every if every % 2 else every + 1)
+        lat.sync()
+        t1 = time.perf_counter()
+        lat.step(1 + warmup, steps)
+        lat.sync()
+        t2 = time.perf_counter()
+        kern_ms, launches = lat.get_timing()
+        lat.set_timing(False)
+        st = lat.status()
+        ms_step = (t2 - t1) * 1e3 / steps
+        ncell = prm.ncell
+        mlups = ncell * steps / (t2 - t1) / 1e6
+        avg_kern = kern_ms / max(launches, 1)
+        bpc = BYTES_PER_CELL[precision]
+        achieved = bpc * ncell / (avg_kern * 1e-3) / 1e9
+        step_gbs = bpc * ncell / (ms_step * 1e-3) / 1e9
+        log(f"{tag}: {ms_step:.3f} ms/step, {mlups:.0f} MLUPS, collide {avg_kern:.3f} ms "
+            f"({achieved:.0f} GB/s), step {step_gbs:.0f} GB/s, links {st['nlinks']}, "
+            f"contact links {st['contact_links']}, refilled nodes {st['refill']}")
+        traffic = MEASURED_TRAFFIC.get((case_name, precision, storage))
+        out.update({
+            "mlups": mlups, "ms_per_step": ms_step, "kernel_ms": avg_kern, "launches": launches,
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": traffic, "traffic_source": TRAFFIC_SOURCE if traffic else None,
+                         "kernel": KERNEL[precision] + ("" if storage == "ab" else " (in-place AA layouts)"),
+                         "bytes_per_cell": bpc,
+                         "step_achieved": round(step_gbs, 1), "step_frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+            "links": st["nlinks"], "contact_links": st["contact_links"], "refilled": st["refill"]})
         done = warmup + steps
+    if collect is not None:
         if total_steps > done:
             lat.step(1 + done, total_steps - done)
             lat.sync()
-        state = collect(lat)
+        out["state"] = collect(lat)
+        out["contact_links_total"] = lat.status()["contact_links"]
         log(f"{tag}: state after {max(total_steps, done)} steps collected")
     lat.close()
-    return {
-        "state": state,
-        "mlups": mlups, "ms_per_step": ms_step, "kernel_ms": avg_kern, "launches": launches,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC.get((case_name, precision, storage)),
-                     "kernel": {"fp32": "k_collide<float,float>", "fp32x": "k_collide<float,double>",
-                                "fp64": "k_collide<double,double>"}[precision]
-                               + ("" if storage == "ab" else " (in-place AA layouts)"),
-                     "bytes_per_cell": BYTES_PER_CELL[precision]},
-        "links": st["nlinks"],
-    }, field
+    return out, field
 
 
-def run_accuracy(case, field, nsteps, modes=("fp32",)):
+def drag_report(Fa, Fb, pos=None, shape=None, rad=None, walls_y=False):
+    """Per-sphere hydrodynamic force Fa against the reference Fb: normwise
+    L2, the per-sphere error relative to the sphere's own |F| and to the rms
+    |F|, and the worst sphere with its surface gap to its nearest neighbour
+    (and to a wall)."""
+    import numpy as np
+    fn = np.linalg.norm(Fb, axis=1)
+    dfn = np.linalg.norm(Fa - Fb, axis=1)
+    frms = float(np.sqrt(np.mean(fn ** 2)))
+    rel = dfn / np.maximum(fn, 1e-300)
+    k = int(np.argmax(rel))
+    out = {"drag_rel_l2": float(np.linalg.norm(Fa - Fb) / np.linalg.norm(Fb)),
+           "drag_max_rel": float(rel.max()), "drag_median_rel": float(np.median(rel)),
+           "drag_max_err_over_rms_drag": float(dfn.max() / frms),
+           "worst_sphere": {"id": k + 1, "rel_err": float(rel[k]), "abs_F_over_rms": float(fn[k] / frms)}}
+    if pos is not None:
+        d = pos - pos[k]
+        L = np.array(shape, dtype=float)[::-1]
+        for a in range(3):
+            if a == 1 and walls_y:
+                continue
+            d[:, a] -= L[a] * np.round(d[:, a] / L[a])
+        dist = np.linalg.norm(d, axis=1)
+        dist[k] = np.inf
+        out["worst_sphere"]["min_surface_gap"] = float(dist.min() - 2 * rad)
+        if walls_y:
+            out["worst_sphere"]["wall_gap"] = float(min(pos[k, 1], L[1] - pos[k, 1]) - rad)
+    return out
+
+
+def run_accuracy(case, field, nsteps, modes=("fp32",), control=6e-8):
     """FP32-storage paths against the plain FP64 path (the drop-in, which
     matches the FP64 oracle to <= 1e-12 per step) after nsteps from the same
     state: velocity relative L2, the statistc energy spectrum and the
-    per-particle hydrodynamic force."""
+    per-particle hydrodynamic force.  With `control` > 0, an FP64 run from an
+    initial field perturbed by that relative amount (the size of one FP32
+    rounding) is compared the same way: how far the flow itself carries a
+    perturbation of that size."""
     import numpy as np
     from multiphase_flow_codes_amd.workloads import setup_lattice
     out = {}
-    for prec in ("fp64",) + tuple(modes):
-        lat, prm, _ = setup_lattice(case, prec, field)
+    runs = [("fp64", 0.0)] + [(m, 0.0) for m in modes] + ([("fp64", control)] if control > 0 else [])
+    for prec, pert in runs:
+        key = prec if pert == 0 else "fp64_perturbed"
+        lat, prm, _ = setup_lattice(case, prec, field if pert == 0 else None, perturb=pert)
         lat.step(1, nsteps)
         lat.sync()
         _, ux, uy, uz = lat.download_macro()
         nek = case.n // 2 - 1
-        out[prec] = {"u": np.stack([ux, uy, uz]), "ek": lat.diag_spectrum(nek),
-                     "f": lat.prt_get("fHIp")[: case.npart], "sums": lat.diag_sums()}
+        out[key] = {"u": np.stack([ux, uy, uz]), "ek": lat.diag_spectrum(nek),
+                    "f": lat.prt_get("fHIp")[: case.npart], "pos": lat.prt_get("ypglb")[: case.npart],
+                    "sums": lat.diag_sums(), "contact": lat.status()["contact_links"]}
         lat.close()
-        log(f"accuracy: {prec} {nsteps} steps done")
+        log(f"accuracy: {key} {nsteps} steps done")
     b = out["fp64"]
     res = {}
-    for mode in modes:
+    for mode in [m for m in out if m != "fp64"]:
         a = out[mode]
         du = float(np.linalg.norm(a["u"] - b["u"]) / np.linalg.norm(b["u"]))
         ek_rel = np.abs(a["ek"] - b["ek"]) / np.maximum(b["ek"], 1e-300)
-        fn = np.linalg.norm(b["f"], axis=1)
-        dfn = np.linalg.norm(a["f"] - b["f"], axis=1)
-        frms = float(np.sqrt(np.mean(fn ** 2)))
         res[mode] = {
             "steps": nsteps, "reference": "plain FP64 GPU path",
             "velocity_rel_l2": du,
             "spectrum_max_rel": float(ek_rel.max()),
             "spectrum_energy_weighted_rel": float(np.abs(a["ek"] - b["ek"]).sum() / b["ek"].sum()),
-            "drag_rel_l2": float(np.linalg.norm(a["f"] - b["f"]) / np.linalg.norm(b["f"])),
-            "drag_max_rel": float((dfn / np.maximum(fn, 1e-300)).max()),
-            "drag_max_err_over_rms_drag": float(dfn.max() / frms),
-            "drag_median_rel": float(np.median(dfn / np.maximum(fn, 1e-300))),
-            "kinetic_energy_rel": float(abs(a["sums"]["u2"] - b["sums"]["u2"]) / b["sums"]["u2"])}
+            "kinetic_energy_rel": float(abs(a["sums"]["u2"] - b["sums"]["u2"]) / b["sums"]["u2"]),
+            "contact_links_total": a["contact"]}
+        res[mode].update(drag_report(a["f"], b["f"], b["pos"], case.shape, case.rad, case.walls))
+        if mode == "fp64_perturbed":
+            res[mode]["perturbation"] = f"initial mode amplitudes x (1 + {control:g} xi), xi ~ N(0,1)"
         log(f"accuracy {mode}: {res[mode]}")
     return res
 
@@ -192,12 +240,13 @@ def run_serving_bench(ncases, cap, rate, window=500, tol=1e-4):
 
 def _channel_state(lat):
     """Velocity (kept in FP32 on the host: 13 GB at 1024^3), the statistc
-    spectrum and the per-sphere hydrodynamic force."""
+    spectrum and the per-sphere hydrodynamic force and position."""
     import numpy as np
     _, ux, uy, uz = lat.download_macro()
     u = [a.astype(np.float32) for a in (ux, uy, uz)]
     del ux, uy, uz
-    return {"u": u, "ek": lat.diag_spectrum(lat.prm.nx // 2 - 1), "f": lat.prt_get("fHIp")}
+    return {"u": u, "ek": lat.diag_spectrum(lat.prm.nx // 2 - 1), "f": lat.prt_get("fHIp"),
+            "pos": lat.prt_get("ypglb")}
 
 
 def _sq_sum(arrs, other=None, chunk=1 << 26):
@@ -215,16 +264,18 @@ def _sq_sum(arrs, other=None, chunk=1 << 26):
     return s
 
 
-def run_channel(steps, warmup, fp32_aa=False, fp64=True, acc_steps=500):
+def run_channel(steps, warmup, fp32_aa=False, fp64=True, acc_steps=500, control=6e-8):
     """The 1024^3 channel with 20,000 spheres: FP32 shifted populations with
     two buffers (163 GB) beside FP64 with one buffer updated in place
     (AA pattern, 163 GB); two FP64 buffers would need 326 GB.  With acc_steps
     both runs continue to that step and the FP32 velocity, spectrum and
-    sphere drag are compared with the FP64 run's."""
+    sphere drag are compared with the FP64 run's; with `control` an FP64 run
+    from an initial field perturbed by that relative amount is compared the
+    same way."""
     import numpy as np
     from multiphase_flow_codes_amd import workloads
     case = workloads.CHANNEL1024
-    keep = ("mlups", "ms_per_step", "kernel_ms", "roofline", "links")
+    keep = ("mlups", "ms_per_step", "kernel_ms", "roofline", "links", "contact_links", "refilled")
     acc = acc_steps > 0 and fp64
     col = _channel_state if acc else None
     r32, _ = run_gpu(case, "fp32", steps, warmup, None, "channel1024", total_steps=acc_steps, collect=col)
@@ -240,37 +291,67 @@ def run_channel(steps, warmup, fp32_aa=False, fp64=True, acc_steps=500):
         out["fp64"] = {k: r64[k] for k in keep} | {"storage": "one buffer in place (AA), 163 GB"}
         out["fp32_over_fp64"] = round(r32["mlups"] / r64["mlups"], 3)
     if acc:
-        a, b = r32["state"], r64["state"]
-        du = float(np.sqrt(_sq_sum(a["u"], b["u"]) / _sq_sum(b["u"])))
-        fn = np.linalg.norm(b["f"], axis=1)
-        dfn = np.linalg.norm(a["f"] - b["f"], axis=1)
-        ek = np.abs(a["ek"] - b["ek"]) / np.maximum(b["ek"], 1e-300)
-        out["accuracy_fp32_vs_fp64"] = {
-            "steps": acc_steps, "reference": "FP64 run of the same case (in-place storage)",
-            "velocity_rel_l2": du, "note": "velocities compared as FP32 copies (6e-8 relative)",
-            "spectrum_max_rel": float(ek.max()),
-            "drag_rel_l2": float(np.linalg.norm(a["f"] - b["f"]) / np.linalg.norm(b["f"])),
-            "drag_median_rel": float(np.median(dfn / np.maximum(fn, 1e-300))),
-            "drag_max_err_over_rms_drag": float(dfn.max() / np.sqrt(np.mean(fn ** 2)))}
-        log(f"channel accuracy: {out['accuracy_fp32_vs_fp64']}")
+        b = r64["state"]
+        legs = [("accuracy_fp32_vs_fp64", r32["state"])]
+        if control > 0:
+            rp, _ = run_gpu(case, "fp64", 0, 0, None, "channel1024", storage="aa", total_steps=acc_steps,
+                            collect=col, perturb=control)
+            legs.append(("control_fp64_perturbed_vs_fp64", rp["state"]))
+        for name, a in legs:
+            du = float(np.sqrt(_sq_sum(a["u"], b["u"]) / _sq_sum(b["u"])))
+            ek = np.abs(a["ek"] - b["ek"]) / np.maximum(b["ek"], 1e-300)
+            out[name] = {
+                "steps": acc_steps, "reference": "FP64 run of the same case (in-place storage)",
+                "velocity_rel_l2": du, "note": "velocities compared as FP32 copies (6e-8 relative)",
+                "spectrum_max_rel": float(ek.max())}
+            out[name].update(drag_report(a["f"], b["f"], b["pos"], case.shape, case.rad, True))
+            if name.startswith("control"):
+                out[name]["perturbation"] = f"initial mode amplitudes x (1 + {control:g} xi), xi ~ N(0,1)"
+            log(f"channel {name}: {out[name]}")
     return out
 
 
-def run_cpu_baseline(case, field, nsteps):
+def cpu_info():
+    model = None
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"cpu_model": model, "nproc": avail, "cpu_count": os.cpu_count()}
+
+
+def run_cpu_baseline(nsteps):
+    """The FP64 C oracle (reference single-rank step order, OpenMP over the
+    fluid loops) on channel256, a 256^3 section of the headline channel with
+    its solid fraction, for nsteps steps from the same kind of initial state."""
+    import numpy as np
     from oracle import pyoracle
+    from multiphase_flow_codes_amd import workloads
+    from multiphase_flow_codes_amd.initial import random_spheres
     import subprocess
     if not os.path.exists(pyoracle.LIB_PATH):
         subprocess.check_call(["bash", os.path.join(ROOT, "oracle", "build.sh")])
+    case = workloads.CHANNEL256
     prm = case.params()
     o = pyoracle.Oracle(prm, bb_mode=0)
-    o.ux[:], o.uy[:], o.uz[:] = case.host_field() if field is None else field
+    o.ux[:], o.uy[:], o.uz[:] = case.host_field()
     o.initpop()
     o.rhoupdat()
     o.macrovar()
-    o.S.istep00 = 0
-    o.initpartpos()
-    o.S.istep00 = 1
+    pos = random_spheres(case.npart, case.rad, case.shape, gap=prm.mingap, wall_clear=case.wall_clear,
+                         walls_y=True, seed=prm.iseedp)
+    o.ypglb[:] = pos
+    o.yp[:] = pos
     o.wp[:] = 0.0
+    o.omgp[:] = 0.0
+    o.S.istep00 = 1
     o.beads_links()
     nthreads = pyoracle.lib().orc_nthreads()
     t0 = time.perf_counter()
@@ -281,8 +362,9 @@ def run_cpu_baseline(case, field, nsteps):
     mlups = prm.ncell * nsteps / dt / 1e6
     log(f"cpu oracle: {nsteps} steps in {dt:.1f} s -> {mlups:.2f} MLUPS on {nthreads} threads")
     return {"value": round(mlups, 3), "unit": "MLUPS", "cores": nthreads, "kind": "port",
-            "sample": f"{nsteps} steps of the same {case.n}^3 / {case.npart}-sphere case, FP64 C "
-                      f"oracle (reference single-rank order), OpenMP on the fluid loops"}
+            "sample": f"{nsteps} steps ({dt:.1f} s) of {case.describe()} -- a 256^3 section of the "
+                      f"headline channel with its solid fraction; FP64 C oracle in the reference's "
+                      f"single-rank step order, OpenMP over the fluid loops"} | cpu_info()
 
 
 def calibrate_copy(mib):
@@ -334,14 +416,26 @@ def hbm_ceilings(gib=4, reps=5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--case", default="box256", choices=["box64", "box256"])
+    ap.add_argument("--steps", type=int, default=100, help="timed steps of every throughput leg")
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--channel-accuracy-steps", type=int, default=500,
+                    help="run the channel legs to this step and compare FP32 with FP64 (0: skip)")
+    ap.add_argument("--control", type=float, default=6e-8,
+                    help="relative perturbation of the FP64 control runs (0: no control runs)")
+    ap.add_argument("--channel-fp32-aa", action="store_true",
+                    help="also run the 1024^3 channel in FP32 with in-place storage")
+    ap.add_argument("--channel-no-fp64", action="store_true")
+    ap.add_argument("--channel-only", action="store_true",
+                    help="run only the 1024^3 channel legs (profiling); prints their JSON on stderr")
+    ap.add_argument("--no-box256", action="store_true")
+    ap.add_argument("--box256-only", action="store_true",
+                    help="run only the 256^3 box legs (profiling); prints their JSON on stderr")
     ap.add_argument("--accuracy-steps", type=int, default=1000,
-                    help="steps of the FP32-vs-FP64 accuracy comparison (0: skip)")
-    ap.add_argument("--no-fp64", action="store_true")
+                    help="steps of the 256^3 FP32-vs-FP64 accuracy comparison (0: skip)")
+    ap.add_argument("--no-fp32x", action="store_true",
+                    help="skip the FP32-storage / FP64-collision variant of the 256^3 box")
     ap.add_argument("--ens-cases", type=int, default=96)
     ap.add_argument("--ens-steps", type=int, default=2000)
     ap.add_argument("--serve-cases", type=int, default=24)
@@ -350,19 +444,6 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=8.0, help="arrivals per second")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--no-fp32x", action="store_true",
-                    help="skip the FP32-storage / FP64-collision variant")
-    ap.add_argument("--no-fp32", action="store_true",
-                    help="skip the FP32 path (profiling runs of the FP64 kernels only)")
-    ap.add_argument("--channel-steps", type=int, default=100,
-                    help="steps of the 1024^3 channel leg (FP32 A/B and FP64 AA; 0: skip)")
-    ap.add_argument("--channel-accuracy-steps", type=int, default=500,
-                    help="run both channel legs to this step and compare FP32 with FP64 (0: skip)")
-    ap.add_argument("--channel-only", action="store_true",
-                    help="run only the 1024^3 channel leg (profiling)")
-    ap.add_argument("--channel-fp32-aa", action="store_true",
-                    help="also run the 1024^3 channel in FP32 with in-place storage")
-    ap.add_argument("--channel-no-fp64", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized buffer 5 times (HBM counter calibration)")
     ap.add_argument("--ceilings", action="store_true",
@@ -379,77 +460,86 @@ def main():
                                    "(z-slab halos) is designed but not implemented",
                           "n_gpus": args.gpus}))
         sys.exit(2)
-    from multiphase_flow_codes_amd import workloads
+    box = None
+    if not args.channel_only and not args.no_box256:
+        box = run_box256(args)
+        if args.box256_only:
+            log(json.dumps(box))
+            return
+    c3 = run_channel(args.steps, args.warmup, args.channel_fp32_aa, not args.channel_no_fp64,
+                     args.channel_accuracy_steps, args.control)
     if args.channel_only:
-        log(json.dumps(run_channel(args.channel_steps, min(args.warmup, 10), args.channel_fp32_aa,
-                                   not args.channel_no_fp64, args.channel_accuracy_steps)))
+        log(json.dumps(c3))
         return
-    case = workloads.WORKLOADS[args.case]
-    r32 = r64 = field = None
-    if not args.no_fp32:
-        r32, field = run_gpu(case, "fp32", args.steps, args.warmup, None, args.case)
-    if not args.no_fp64:
-        r64, field = run_gpu(case, "fp64", args.steps, args.warmup, field, args.case)
-    r32x = None
-    if not args.no_fp32x:
-        r32x, field = run_gpu(case, "fp32x", args.steps, args.warmup, field, args.case)
-    if r32 is None:
-        log("FP32 path skipped: no headline line printed")
-        return
-    acc = None
-    if args.accuracy_steps > 0 and not args.no_fp64:
-        acc = run_accuracy(case, field, args.accuracy_steps,
-                           ("fp32",) if args.no_fp32x else ("fp32", "fp32x"))
     ens = None if args.no_ensemble else run_ensemble_bench(args.ens_cases, args.ens_steps)
     srv = None if args.no_serving else run_serving_bench(args.serve_cases, args.serve_cap, args.serve_rate)
-    c3 = (run_channel(args.channel_steps, min(args.warmup, 10), args.channel_fp32_aa, not args.channel_no_fp64,
-                      args.channel_accuracy_steps)
-          if args.channel_steps > 0 else None)
-    cpu = None
-    if not args.no_cpu:
-        cpu = run_cpu_baseline(case, field, args.cpu_steps)
+    cpu = None if args.no_cpu else run_cpu_baseline(args.cpu_steps)
+    f32 = c3["fp32"]
     out = {
         "metric": "MLUPS + %HBM roofline, FP32 vs FP64 path; p50/p99 case latency (mixed batch)",
-        "value": round(r32["mlups"], 1),
+        "value": round(f32["mlups"], 1),
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(r32["ms_per_step"], 4),
+        "ms_per_step": round(f32["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": case.describe(), "cells": case.n ** 3, "particles": case.npart,
-                   "boundary_links": r32["links"], "populations": "FP32 shifted deviations",
+        "config": {"workload": c3["workload"], "cells": c3["cells"], "particles": c3["particles"],
+                   "boundary_links": f32["links"], "contact_links": f32["contact_links"],
+                   "populations": "FP32 shifted deviations, two buffers (A/B)",
                    "particle_arithmetic": "f64"},
-        "roofline": r32["roofline"],
+        "roofline": f32["roofline"],
         "cpu_baseline": cpu,
     }
-    if r64 is not None:
-        out["fp64"] = {"value": round(r64["mlups"], 1), "unit": "MLUPS",
-                       "ms_per_step": round(r64["ms_per_step"], 4), "roofline": r64["roofline"]}
-        out["fp32_over_fp64"] = round(r32["mlups"] / r64["mlups"], 3)
-    if r32x is not None:
-        out["fp32_storage_fp64_collision"] = {
-            "value": round(r32x["mlups"], 1), "unit": "MLUPS", "ms_per_step": round(r32x["ms_per_step"], 4),
-            "roofline": r32x["roofline"]}
-        if r64 is not None:
-            out["fp32_storage_fp64_collision"]["over_fp64"] = round(r32x["mlups"] / r64["mlups"], 3)
+    if "fp64" in c3:
+        f64 = c3["fp64"]
+        out["fp64"] = {"value": round(f64["mlups"], 1), "unit": "MLUPS",
+                       "ms_per_step": round(f64["ms_per_step"], 4), "roofline": f64["roofline"],
+                       "storage": f64["storage"], "contact_links": f64["contact_links"]}
+        out["fp32_over_fp64"] = c3["fp32_over_fp64"]
+    for k in ("accuracy_fp32_vs_fp64", "control_fp64_perturbed_vs_fp64", "fp32_aa"):
+        if k in c3:
+            out[k] = c3[k]
+    if box is not None:
+        out["box256"] = box
     if ens is not None:
         out["ensemble"] = ens
     if srv is not None:
         out["serving"] = srv
         out["p50_case_latency_s"] = srv["engine_fp32"]["p50_s"]
         out["p99_case_latency_s"] = srv["engine_fp32"]["p99_s"]
-    if c3 is not None:
-        out["channel1024"] = c3
-    if acc is not None:
+    print(json.dumps(out), flush=True)
+
+
+def run_box256(args):
+    """Config 2: the 256^3 periodic box with 300 spheres, FP32, FP64 and
+    FP32-storage/FP64-collision throughput, and their accuracy against FP64
+    after args.accuracy_steps steps with the perturbed-FP64 control."""
+    from multiphase_flow_codes_amd import workloads
+    case = workloads.BOX256
+    keep = ("mlups", "ms_per_step", "kernel_ms", "roofline", "links", "contact_links", "refilled")
+    r32, field = run_gpu(case, "fp32", args.steps, args.warmup, None, "box256")
+    r64, field = run_gpu(case, "fp64", args.steps, args.warmup, field, "box256")
+    out = {"workload": case.describe(), "cells": case.n ** 3, "particles": case.npart,
+           "fp32": {k: r32[k] for k in keep}, "fp64": {k: r64[k] for k in keep},
+           "fp32_over_fp64": round(r32["mlups"] / r64["mlups"], 3)}
+    if not args.no_fp32x:
+        r32x, field = run_gpu(case, "fp32x", args.steps, args.warmup, field, "box256")
+        out["fp32_storage_fp64_collision"] = {k: r32x[k] for k in keep}
+        out["fp32_storage_fp64_collision"]["over_fp64"] = round(r32x["mlups"] / r64["mlups"], 3)
+    if args.accuracy_steps > 0:
+        acc = run_accuracy(case, field, args.accuracy_steps,
+                           ("fp32",) if args.no_fp32x else ("fp32", "fp32x"), args.control)
         out["accuracy_fp32_vs_fp64"] = acc["fp32"]
         if "fp32x" in acc:
             out["fp32_storage_fp64_collision"]["accuracy_vs_fp64"] = acc["fp32x"]
-    print(json.dumps(out), flush=True)
+        if "fp64_perturbed" in acc:
+            out["control_fp64_perturbed_vs_fp64"] = acc["fp64_perturbed"]
+    return out
 
 
 if __name__ == "__main__":

@@ -19,7 +19,15 @@
 !                        particles and the forcing-process state at the
 !                        last step (savecntdflow / savecntdpart / input_outputf)
 ! and pushed to the device when the context is created (restart: f, ux..uz,
-! particles) and at the first FORCING (the forcing-process state).
+! particles with the integrator history that savecntdpart / loadcntdpart
+! carry) and at the first FORCING (the forcing-process state).  After the
+! release the node maps ibnodes / isnodes follow the device at the macro
+! cadences above, and ibnodes0 / isnodes0 at the last step (diag, rmsstat,
+! sijstat03 mask by them; savecntdpart writes the previous pair).
+! Precision and storage are read from the environment when the context is
+! created: MFC_PRECISION = fp64 (default) | fp32 | fp32x (FP32 storage, FP64
+! collision); MFC_STORAGE = ab (default, two buffers) | aa (one buffer
+! updated in place).
 ! A library error prints 'fault: <routine> (<code>) <message>' as the
 ! reference's own fatal checks do, then ends the run with a nonzero status.
       module mfc_dropin_state
@@ -29,7 +37,18 @@
       type(c_ptr), save :: ctx = c_null_ptr
       logical, save :: forcing_started = .false.
       logical, save :: created = .false.
+      ! the device holds node maps (beads_links has run on this context)
+      logical, save :: maps_built = .false.
       contains
+
+      ! value of environment variable `name`, or `dflt` when it is unset
+      function env_choice(name, dflt) result(v)
+        character(len=*), intent(in) :: name, dflt
+        character(len=16) :: v
+        integer :: n, st
+        call get_environment_variable(name, v, n, st)
+        if (st /= 0 .or. n == 0) v = dflt
+      end function env_choice
 
       subroutine mfc_check(rc, where)
         integer(c_int), intent(in) :: rc
@@ -79,13 +98,31 @@
       p%nx = nx
       p%ny = ny
       p%nz = nz
-      p%precision = MFC_PREC_FP64
+      select case (trim(env_choice('MFC_PRECISION', 'fp64')))
+      case ('fp64')
+        p%precision = MFC_PREC_FP64
+      case ('fp32')
+        p%precision = MFC_PREC_FP32
+      case ('fp32x')
+        p%precision = MFC_PREC_FP32_F64MATH
+      case default
+        write(*,*) 'fault: MFC_PRECISION must be fp64, fp32 or fp32x'
+        error stop 1
+      end select
       p%npart_max = 0
       if (ipart) p%npart_max = npart
       p%walls_y = 0
       p%iseedf = iseedf
       p%irelease = irelease
-      p%storage = MFC_STORE_AB
+      select case (trim(env_choice('MFC_STORAGE', 'ab')))
+      case ('ab')
+        p%storage = MFC_STORE_AB
+      case ('aa')
+        p%storage = MFC_STORE_AA
+      case default
+        write(*,*) 'fault: MFC_STORAGE must be ab or aa'
+        error stop 1
+      end select
       p%s1 = s1
       p%s2 = s2
       p%s4 = s4
@@ -146,6 +183,13 @@
       call mfc_check(mfc_prt_set(ctx, MFC_PF_WPP, wpp), 'set wpp')
       call mfc_check(mfc_prt_set(ctx, MFC_PF_OMGP, omgp), 'set omgp')
       call mfc_check(mfc_prt_set(ctx, MFC_PF_OMGPP, omgpp), 'set omgpp')
+      ! integrator history of beads_move (BOX/partlib.f90:2332-2365), held
+      ! per rank-local slot; one rank keeps slot ip = particle ip
+      call mfc_check(mfc_prt_set(ctx, MFC_PF_FORCEPP, forcepp), 'set forcepp')
+      call mfc_check(mfc_prt_set(ctx, MFC_PF_TORQPP, torqpp), 'set torqpp')
+      call mfc_check(mfc_prt_set(ctx, MFC_PF_DWDT, dwdt), 'set dwdt')
+      call mfc_check(mfc_prt_set(ctx, MFC_PF_DOMGDT, domgdt), 'set domgdt')
+      call mfc_check(mfc_prt_set(ctx, MFC_PF_THETAP, thetap), 'set thetap')
       end subroutine mfc_push_particles
 !===========================================================================
 
@@ -162,8 +206,33 @@
       call mfc_check(mfc_prt_get(ctx, MFC_PF_FHIP, fHIp), 'get fHIp')
       call mfc_check(mfc_prt_get(ctx, MFC_PF_FLUBP, flubp), 'get flubp')
       call mfc_check(mfc_prt_get(ctx, MFC_PF_TORQP, torqp), 'get torqp')
+      call mfc_check(mfc_prt_get(ctx, MFC_PF_FORCEPP, forcepp), 'get forcepp')
+      call mfc_check(mfc_prt_get(ctx, MFC_PF_TORQPP, torqpp), 'get torqpp')
+      call mfc_check(mfc_prt_get(ctx, MFC_PF_DWDT, dwdt), 'get dwdt')
+      call mfc_check(mfc_prt_get(ctx, MFC_PF_DOMGDT, domgdt), 'get domgdt')
+      call mfc_check(mfc_prt_get(ctx, MFC_PF_THETAP, thetap), 'get thetap')
       yp(:, 1:npart) = ypglb(:, 1:npart)
       end subroutine mfc_pull_particles
+!===========================================================================
+
+! Ends the device context (frees the GPU state); the next wrapper call
+! creates a new one from the host arrays of var_inc, as a restarted run
+! does.  No counterpart in the reference, where the process exit does this.
+      subroutine mfc_dropin_finish
+      use mfc_dropin_state
+      implicit none
+      integer(c_int) :: rc
+      if (.not. created) return
+      rc = mfc_destroy(ctx)
+      ctx = c_null_ptr
+      created = .false.
+      forcing_started = .false.
+      maps_built = .false.
+      if (rc /= MFC_OK) then
+        write(*,*) 'fault: mfc_destroy failed (', rc, ')'
+        error stop 1
+      end if
+      end subroutine mfc_dropin_finish
 !===========================================================================
 
       SUBROUTINE initpop
@@ -245,6 +314,12 @@
                        force_realz), 'download force')
       end if
       if (want_part) call mfc_pull_particles
+      if (ipart .and. maps_built .and. want_macro) then
+        call mfc_check(mfc_get_maps(ctx, 0_c_int32_t, ibnodes, isnodes),   &
+                       'get maps')
+        if (last) call mfc_check(mfc_get_maps(ctx, 1_c_int32_t, ibnodes0, &
+                                 isnodes0), 'get previous maps')
+      end if
       if (last) then
         call mfc_check(mfc_download_f(ctx, f), 'download f')
         call mfc_check(mfc_get_forcing_state(ctx, i1, i2, ivf, b1r, b2r, b3r, &
@@ -295,6 +370,7 @@
       implicit none
       call mfc_dropin_create
       call mfc_check(mfc_beads_links(ctx), 'beads_links')
+      maps_built = .true.
       end subroutine beads_links
 !===========================================================================
 

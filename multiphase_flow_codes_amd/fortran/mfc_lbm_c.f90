@@ -8,7 +8,8 @@
 
       integer(c_int), parameter :: MFC_OK = 0, MFC_ERR_OVERLAP = 4,     &
                                    MFC_ERR_CAPACITY = 5
-      integer(c_int), parameter :: MFC_PREC_FP64 = 0, MFC_PREC_FP32 = 1
+      integer(c_int), parameter :: MFC_PREC_FP64 = 0, MFC_PREC_FP32 = 1, &
+                                   MFC_PREC_FP32_F64MATH = 2
       integer(c_int), parameter :: MFC_STORE_AB = 0, MFC_STORE_AA = 1
       integer(c_int), parameter :: MFC_PF_YPGLB = 0, MFC_PF_YPGLBP = 1, &
         MFC_PF_WP = 2, MFC_PF_WPP = 3, MFC_PF_OMGP = 4, MFC_PF_OMGPP = 5,&

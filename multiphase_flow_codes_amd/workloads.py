@@ -9,6 +9,8 @@
                decaying (no forcing), nu = 0.002, 20,000 spheres r = 6,
                density ratio 2.5, k0 = 8, u_rms 0.03, sphere surfaces at least
                8 nodes from the walls
+  channel256:  a 256^3 section of channel1024 with its solid fraction (312
+               spheres), for samples of that case on the host
 Forcing acts on the |k| <= sqrt(2) shell with the reference's
 Ornstein-Uhlenbeck mode process (gaussian.f90:182-372).  Spheres start from
 rest at seeded non-overlapping positions (min surface gap 2): the reference's
@@ -76,15 +78,27 @@ CHANNEL1024 = BoxCase("channel1024", 1024, 0.002, 20000, 6.0, 2.5, 8.0, 0.03, wa
 WORKLOADS = {c.name: c for c in (BOX64, BOX256, CHANNEL1024)}
 
 
-def setup_lattice(case: BoxCase, precision: str, field=None, storage: str = "ab"):
+# a 256^3 section of the channel: the same walls, viscosity, sphere radius,
+# density ratio and solid fraction (312 spheres), for host-side samples
+CHANNEL256 = BoxCase("channel256", 256, 0.002, 312, 6.0, 2.5, 8.0, 0.03, walls=True, wall_clear=8.0)
+
+
+def setup_lattice(case: BoxCase, precision: str, field=None, storage: str = "ab",
+                  perturb: float = 0.0):
     """A Lattice holding the case's initial state, spheres placed, ready to
     step from istep = 1 (particles already released).  `field` (ux, uy, uz
-    on the host) replaces the device synthesis of the initial velocity."""
+    on the host) replaces the device synthesis of the initial velocity.
+    `perturb` > 0 multiplies every mode amplitude of the synthesised field by
+    (1 + perturb * xi), xi standard normal (seed 7): a control run whose
+    initial state differs from the case's at that relative level."""
     from .lbm import Lattice
     prm = case.params()
     lat = Lattice(prm, precision, storage=storage)
     if field is None:
         table, scale = solenoidal_table(case.shape, k0=case.k0, u_rms=case.u_rms, seed=IFLOWSEED_REF)
+        if perturb > 0.0:
+            xi = np.random.default_rng(7).standard_normal(table.shape)
+            table = table * (1.0 + perturb * xi)
         lat.init_velocity(table, scale)
     else:
         ux, uy, uz = field

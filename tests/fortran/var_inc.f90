@@ -18,7 +18,7 @@
 
       integer istep, istep0, istep00, nsteps
       integer lz, ly
-      integer iseedp, nps, iyp
+      integer iseedp, msize, nps, iyp
       logical ipart
 
       real pi2, vscale, tscale
@@ -36,13 +36,19 @@
       real,allocatable,dimension(:,:,:):: force_realx,force_realy,     &
                                           force_realz
 
+      integer,allocatable,dimension(:,:,:):: ibnodes, ibnodes0
+      integer,allocatable,dimension(:,:,:):: isnodes, isnodes0
+
       real,allocatable,dimension(:,:,:,:):: f
       real,allocatable,dimension(:,:,:):: rho, rhop
       real,allocatable,dimension(:,:,:):: ux, uy, uz
       real,allocatable,dimension(:,:):: yp, ypglb, ypglbp
       real,allocatable,dimension(:,:):: wp, wpp
       real,allocatable,dimension(:,:):: omgp, omgpp
-      real,allocatable,dimension(:,:):: torqp
+      real,allocatable,dimension(:,:):: dwdt, domgdt
+      real,allocatable,dimension(:,:):: forcepp
+      real,allocatable,dimension(:,:):: torqp, torqpp
+      real,allocatable,dimension(:,:):: thetap
       real,allocatable,dimension(:,:):: fHIp
       real,allocatable,dimension(:,:):: flubp
 

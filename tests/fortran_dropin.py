@@ -36,13 +36,13 @@ def _fmt(v):
     return repr(float(v)).replace("e", "d") if "e" in repr(float(v)) else repr(float(v)) + "d0"
 
 
-def write_inputs(workdir, prm, u, nsteps, nrelax, rhoepsl=1e-30):
+def write_inputs(workdir, prm, u, nsteps, nrelax, rhoepsl=1e-30, nsplit=0):
     vals = {n: getattr(prm, n) for n in ("s1", "s2", "s4", "s9", "s10", "s13", "s16", "omegepsl",
                                          "omegepslj", "omegxx", "ww0", "ww1", "ww2", "volp", "amp",
                                          "aip", "gscale", "rhog", "stf0", "stf1", "vscale",
                                          "tscale", "pi2")}
     vals.update(iseedf=prm.iseedf, iseedp=prm.iseedp, rhoepsl=rhoepsl, nsteps=nsteps,
-                nrelax=nrelax)
+                nrelax=nrelax, nsplit=nsplit)
     with open(os.path.join(workdir, "dropin_in.nml"), "w") as fh:
         fh.write("&run\n")
         for k, v in vals.items():
@@ -52,26 +52,36 @@ def write_inputs(workdir, prm, u, nsteps, nrelax, rhoepsl=1e-30):
         os.path.join(workdir, "dropin_u.bin"))
 
 
-def run_driver(workdir, timeout=120):
-    return subprocess.run([DRIVER], cwd=workdir, capture_output=True, text=True, timeout=timeout)
+def run_driver(workdir, timeout=120, env=None):
+    full = dict(os.environ)
+    full.update(env or {})
+    return subprocess.run([DRIVER], cwd=workdir, capture_output=True, text=True, timeout=timeout,
+                          env=full)
+
+
+PART_OUT = ("ypglb", "wp", "omgp", "fHIp", "torqp", "flubp",
+            "forcepp", "torqpp", "dwdt", "domgdt", "thetap")
 
 
 def read_outputs(workdir, prm):
     raw = open(os.path.join(workdir, "dropin_out.bin"), "rb").read()
     ncell = prm.ncell
-    nd = 19 * ncell + 4 * ncell + 6 * 3 * NPART
+    nd = 19 * ncell + 4 * ncell + len(PART_OUT) * 3 * NPART
     d = np.frombuffer(raw[:8 * nd], dtype="<f8")
     ints = np.frombuffer(raw[8 * nd:], dtype="<i4")
+    assert ints.size == 3 + 4 * ncell, ints.size
     shp = (prm.nz, prm.ny, prm.nx)
     out = {"f": d[:19 * ncell].reshape(shp + (19,))}
     o = 19 * ncell
     for name in ("rho", "ux", "uy", "uz"):
         out[name] = d[o:o + ncell].reshape(shp)
         o += ncell
-    for name in ("ypglb", "wp", "omgp", "fHIp", "torqp", "flubp"):
+    for name in PART_OUT:
         out[name] = d[o:o + 3 * NPART].reshape(NPART, 3)
         o += 3 * NPART
     out["iseedf"], out["iyf"], out["nrelax"] = (int(x) for x in ints[:3])
+    for k, name in enumerate(("ibnodes", "isnodes", "ibnodes0", "isnodes0")):
+        out[name] = ints[3 + k * ncell:3 + (k + 1) * ncell].reshape(shp)
     return out
 
 
