@@ -197,17 +197,31 @@ const char* mfc_ens_last_error(const mfc_ens* e);
 int mfc_ens_add(mfc_ens* e, mfc_ctx* box, int32_t istep);
 int mfc_ens_remove(mfc_ens* e, mfc_ctx* box);
 int mfc_ens_size(const mfc_ens* e);
+/* Steps go in chunks of up to 8: the tables of a chunk's steps are built on
+ * the host and uploaded with one copy, then its stage grids run. */
 int mfc_ens_step(mfc_ens* e, int32_t nsteps);
 /* Waits for queued work; returns the first pending fault of any member. */
 int mfc_ens_sync(mfc_ens* e);
-/* Threads per collide block (64, 128 or 256), or chosen by timing each on the
- * current members (they advance 3 * steps steps); ms_per_step3 may be null. */
+/* Threads per collide block (64, 128 or 256): the pencil of consecutive x
+ * nodes one workgroup collides. */
 int mfc_ens_set_block(mfc_ens* e, int32_t bs);
-/* on = 1: each step (its table upload and stage grids) is recorded once as a
- * HIP graph per table buffer and replayed while membership and launch shapes
- * stay the same (re-recorded when they change); on = 0: plain launches. */
+/* Collide tile order: 0 launch order, 1 blocks dealt to the 8 XCDs in
+ * contiguous ranges (results do not depend on it). */
+int mfc_ens_set_tile_order(mfc_ens* e, int32_t order);
+/* on = 1: each chunk's stage grids (kernel nodes only; the table upload
+ * stays a plain copy) are recorded once as a HIP graph and replayed for
+ * every chunk of the same launch shapes (a few recordings are kept); on = 0:
+ * plain launches. */
 int mfc_ens_set_graph(mfc_ens* e, int32_t on);
-int mfc_ens_autotune(mfc_ens* e, int32_t steps, int32_t* chosen, double* ms_per_step3);
+/* Times `steps` steps of every (block size, tile order) pair after `steps`
+ * untimed ones, on the current members (they advance 12 * steps steps), and
+ * keeps the fastest: chosen[0] = block size, chosen[1] = tile order;
+ * ms_per_step6[k] = time of block {64,128,256}[k % 3] with order k / 3.
+ * chosen and ms_per_step6 may be null. */
+int mfc_ens_autotune(mfc_ens* e, int32_t steps, int32_t* chosen, double* ms_per_step6);
+/* Host time spent building and enqueueing steps since the last call (waits
+ * for table buffers excluded) and the number of steps it covers; resets both. */
+int mfc_ens_host_time(mfc_ens* e, double* ms, int64_t* steps);
 void* mfc_ens_get_stream(mfc_ens* e);
 
 /* Collide-kernel timing with HIP events on the library's stream: every = N

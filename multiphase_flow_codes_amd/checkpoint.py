@@ -86,27 +86,66 @@ def write_record(fh, arrays, max_sub: int = MAX_SUBRECORD):
         fh.write(struct.pack("<i", -n if s > 0 else n))
 
 
-def read_record(fh) -> bytes:
-    """The payload of the next record (subrecords joined)."""
+def read_record(fh, expect: int | None = None) -> np.ndarray:
+    """The payload of the next record as one uint8 array, each subrecord read
+    straight into it (no second copy).  `expect`, when given, is the payload
+    length the caller needs (preallocated; a different length is an error).
+
+    Two framings are read: subrecords of at most 2^31 - 9 bytes whose markers
+    carry the sign of continuation (the GNU and Intel runtimes' records over
+    2 GB, as written here), and a record of 2^31 bytes or more framed by a
+    single pair of markers holding its length as an unsigned 32-bit value
+    (the runtime of amdflang).  A negative first marker is either; the
+    trailing marker tells them apart: +|n| after |n| bytes for a first
+    subrecord, the same negative value after 2^32 + n bytes otherwise."""
+    buf = np.empty(expect, dtype=np.uint8) if expect is not None else None
     parts = []
+    pos = 0
     first = True
     while True:
         head = fh.read(4)
         if len(head) < 4:
             raise EOFError("end of file inside a record" if not first else "end of file")
-        (n,) = struct.unpack("<i", head)
-        more = n < 0
-        n = abs(n)
-        data = fh.read(n)
-        (tail,) = struct.unpack("<i", fh.read(4))
-        if len(data) != n or abs(tail) != n:
+        (h,) = struct.unpack("<i", head)
+        n, more = abs(h), h < 0
+        if h < 0 and first:
+            start = fh.tell()
+            fh.seek(start + n)
+            t = fh.read(4)
+            if len(t) < 4 or struct.unpack("<i", t)[0] != n:
+                # one record with an unsigned 32-bit length (amdflang)
+                fh.seek(start + h + (1 << 32))
+                t = fh.read(4)
+                if len(t) < 4 or struct.unpack("<i", t)[0] != h:
+                    raise ValueError("corrupt record framing")
+                n, more = h + (1 << 32), False
+            fh.seek(start)
+        if buf is not None:
+            if pos + n > buf.size:
+                raise ValueError(f"record longer than the expected {expect} bytes")
+            got = fh.readinto(memoryview(buf)[pos:pos + n])
+        else:
+            data = fh.read(n)
+            got = len(data)
+            parts.append(data)
+        pos += got
+        t = fh.read(4)
+        if got != n or len(t) < 4:
             raise ValueError("corrupt record framing")
-        if (tail < 0) == first:
-            raise ValueError("subrecord continuation markers do not match")
-        parts.append(data)
+        (tail,) = struct.unpack("<i", t)
+        if n < (1 << 31):
+            if abs(tail) != n:
+                raise ValueError("corrupt record framing")
+            if (tail < 0) == first:
+                raise ValueError("subrecord continuation markers do not match")
         first = False
         if not more:
-            return b"".join(parts)
+            break
+    if buf is None:
+        return np.frombuffer(b"".join(parts), dtype=np.uint8)
+    if pos != buf.size:
+        raise ValueError(f"record holds {pos} bytes, expected {expect}")
+    return buf
 
 
 def read_records(path):
@@ -121,15 +160,16 @@ def read_records(path):
                 raise
 
 
-def _split(buf: bytes, spec):
-    """Cut a record into arrays: spec = [(dtype, count), ...]."""
+def _split(buf, spec):
+    """Cut a record into arrays (views of it): spec = [(dtype, count), ...]."""
+    buf = np.asarray(buf, dtype=np.uint8) if not isinstance(buf, np.ndarray) else buf
     out, off = [], 0
     for dt, n in spec:
         a = np.frombuffer(buf, dtype=dt, count=n, offset=off)
-        out.append(a.copy())
+        out.append(a)
         off += a.nbytes
-    if off != len(buf):
-        raise ValueError(f"record holds {len(buf)} bytes, expected {off}")
+    if off != buf.size:
+        raise ValueError(f"record holds {buf.size} bytes, expected {off}")
     return out
 
 
@@ -142,13 +182,14 @@ def save_flow(path, f, nsteps: int, istat: int = 0, imovie: int = 0, max_sub: in
 
 
 def load_flow(path, shape):
-    """(nsteps, istat, imovie), f as (nz, ny, nx, 19)."""
+    """(nsteps, istat, imovie), f as (nz, ny, nx, 19).  The population
+    record is read straight into the returned array (one copy in memory)."""
     nz, ny, nx = shape
-    recs = read_records(path)
-    if len(recs) != 2:
-        raise ValueError(f"{path}: {len(recs)} records, expected 2")
-    (hdr,) = _split(recs[0], [("<i4", 3)])
-    (f,) = _split(recs[1], [("<f8", 19 * nx * ny * nz)])
+    with open(path, "rb") as fh:
+        (hdr,) = _split(read_record(fh, 12), [("<i4", 3)])
+        (f,) = _split(read_record(fh, 8 * 19 * nx * ny * nz), [("<f8", 19 * nx * ny * nz)])
+        if fh.read(1):
+            raise ValueError(f"{path}: more than 2 records")
     return tuple(int(v) for v in hdr), f.reshape(nz, ny, nx, 19)
 
 

@@ -234,17 +234,21 @@ int ensure_macro(mfc_ctx* ctx) {
 }
 
 // Make the rho/u arrays hold the reference's values (materialising a
-// virtual macrovar if needed).
+// virtual macrovar if needed).  A materialised macrovar stays virtual: the
+// arrays are a copy for the caller, and the next collide still recomputes
+// the moments itself, so a step's result (in FP32 arithmetic too) does not
+// depend on whether the host looked at rho/u before it.
 int need_arrays(mfc_ctx* ctx) {
   Ctx& c = ctx->c;
   if (c.mom == MOM_ARRAYS) return MFC_OK;
   if (c.mom == MOM_VIRTUAL) {
     if (c.mom_version != c.f_version)
       return fail(ctx, MFC_ERR_STATE, "rho/u requested after the field changed since macrovar");
+    if (c.rho && c.arrays_version == c.mom_version) return MFC_OK;
     int rc = ensure_macro(ctx);
     if (rc) return rc;
     HIPCHK(launch_macro(c, c.phase == PHASE_POST_STREAM));
-    c.mom = MOM_ARRAYS;
+    c.arrays_version = c.mom_version;
     return MFC_OK;
   }
   int rc = ensure_macro(ctx);
@@ -288,6 +292,9 @@ int check_faults(mfc_ctx* ctx) {
     return fail(ctx, MFC_ERR_CAPACITY, "fault: too many particles in one pair-search cell (overlapping particles)");
   if (s[ST_NONFINITE] > 0 || s[ST_BOX_FAULT] > 0)
     return fail(ctx, MFC_ERR_NONFINITE, "fault: particle position not finite or outside the box");
+  if (c.overlap_used && s[ST_FAST] > 0)
+    return fail(ctx, MFC_ERR_NONFINITE, "fault: a sphere moved a node or more in one step (the overlapped "
+                                        "step needs less; MFC_OVERLAP=0 runs without it)");
   return MFC_OK;
 }
 
@@ -344,6 +351,10 @@ int mfc_create(const mfc_params* p, mfc_ctx** out) {
   {
     const char* e = getenv("MFC_XCD_REMAP");
     c.xcd_remap = (e && e[0] >= '0' && e[0] <= '2') ? e[0] - '0' : -1;
+  }
+  {
+    const char* e = getenv("MFC_OVERLAP");
+    c.overlap = (e && e[0] == '0') ? 0 : 1;
   }
   {
     // Population planes 512 B further apart than the node count: with a
@@ -429,6 +440,14 @@ int mfc_destroy(mfc_ctx* ctx) {
   F(c.d_status); F(c.d_red); F(c.staging);
   F(c.diag_part); F(c.diag_out); F(c.diag_Z);
   for (int a = 0; a < 3; ++a) F(c.diag_tw[a]);
+  {
+    TileLists& T = c.tiles;
+    F(T.mark); F(T.cnt); F(T.off_near); F(T.off_far); F(T.tot); F(T.near_list); F(T.far_list);
+    if (c.ev_coll) hipEventDestroy(c.ev_coll);
+    if (c.ev_tiles) hipEventDestroy(c.ev_tiles);
+    if (c.ev_part) hipEventDestroy(c.ev_part);
+    if (c.pstream) hipStreamDestroy(c.pstream);
+  }
   if (c.h_status) hipHostFree(c.h_status);
   if (c.h_red) hipHostFree(c.h_red);
   for (auto e : c.tpool) hipEventDestroy(e);
@@ -652,6 +671,7 @@ int mfc_collision_mrt(mfc_ctx* ctx) {
     HIPCHK(launch_collide(c, arrays));
     HIPCHK(hipEventRecord(c.tpool[c.tpool_used + 1], c.stream));
     c.tpool_used += 2;
+    c.ttimed++;
   } else {
     HIPCHK(launch_collide(c, arrays));
   }
@@ -1042,11 +1062,141 @@ int mfc_get_links(mfc_ctx* ctx, int32_t* ilink, int32_t* mlink, double* alink) {
 }
 
 // ---- whole step ---------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+// The particle stream, its events and the tile lists of the overlapped step.
+int ensure_overlap(mfc_ctx* ctx) {
+  Ctx& c = ctx->c;
+  if (c.pstream) return MFC_OK;
+  TileLists& T = c.tiles;
+  T.tpr = (c.g.nx + kTileX - 1) / kTileX;
+  T.ntiles = (uint64_t)T.tpr * (uint64_t)c.g.ny * (uint64_t)c.g.nz;
+  // the particle phase's own reach (stencils 3 sqrt 2 beyond the sphere,
+  // 2 nodes of displacement) and the collide's pull (sqrt 2), with a margin
+  T.reach = c.p.rad + 4.0 * sqrt(2.0) + 2.0 + 1e-6;
+  const size_t nchunk = (size_t)((T.ntiles + kTileChunk - 1) / kTileChunk);
+  if (nchunk > (size_t)1024 * 1024) return fail(ctx, MFC_ERR_ARG, "overlapped step: lattice too large for the tile lists");
+  HIPCHK(hipMalloc(&T.mark, T.ntiles));
+  HIPCHK(hipMemsetAsync(T.mark, 0, T.ntiles, c.stream));
+  HIPCHK(hipMalloc(&T.cnt, sizeof(int32_t) * nchunk));
+  HIPCHK(hipMalloc(&T.off_near, sizeof(uint32_t) * nchunk));
+  HIPCHK(hipMalloc(&T.off_far, sizeof(uint32_t) * nchunk));
+  HIPCHK(hipMalloc(&T.tot, sizeof(uint32_t) * 4));
+  HIPCHK(hipMalloc(&T.near_list, sizeof(uint32_t) * 2 * T.ntiles));
+  HIPCHK(hipMalloc(&T.far_list, sizeof(uint32_t) * 2 * T.ntiles));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  HIPCHK(hipStreamCreateWithFlags(&c.pstream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&c.ev_coll, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&c.ev_tiles, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&c.ev_part, hipEventDisableTiming));
+  return MFC_OK;
+}
+
+// The collide of the far or the near tiles of list set k, timed like
+// mfc_collision_mrt's launch when timing is on.
+int collide_tiles(mfc_ctx* ctx, int k, bool far, bool timed) {
+  Ctx& c = ctx->c;
+  TileLists& T = c.tiles;
+  const uint32_t* list = (far ? T.far_list : T.near_list) + (size_t)k * T.ntiles;
+  const uint32_t* count = T.tot + 2 * k + (far ? 1 : 0);
+  if (timed) HIPCHK(hipEventRecord(c.tpool[c.tpool_used], c.stream));
+  HIPCHK(launch_collide_tiles(c, list, count, far));
+  if (timed) {
+    HIPCHK(hipEventRecord(c.tpool[c.tpool_used + 1], c.stream));
+    c.tpool_used += 2;
+  }
+  return MFC_OK;
+}
+
+// mfc_step with the particle phase of step n on the particle stream, beside
+// the collide of step n + 1 on the tiles far from every sphere; the near
+// tiles follow once that phase is done.  Every kernel computes exactly what
+// it computes in the plain sequence (the tiles only split the collide's
+// grid), so results are bitwise those of the plain step.
+int step_overlapped(mfc_ctx* ctx, int32_t istep, int32_t nsteps) {
+  Ctx& c = ctx->c;
+  Particles& P = c.pt;
+  int rc = ensure_overlap(ctx);
+  if (rc) return rc;
+  c.overlap_used = true;
+  const hipStream_t sA = c.stream, sB = c.pstream;
+  TileLists& T = c.tiles;
+  // step istep: its FORCING and the collide of the whole lattice
+  if ((rc = mfc_forcing(ctx))) return rc;
+  if ((rc = mfc_collision_mrt(ctx))) return rc;
+  for (int s = 0; s < nsteps; ++s) {
+    const int it = istep + s;
+    const bool more = s + 1 < nsteps;
+    const int k = (it + 1) & 1;   // list set of step it + 1's collide
+    HIPCHK(hipEventRecord(c.ev_coll, sA));
+    c.stream = sB;
+    int prc = MFC_OK;
+    hipError_t he = hipSuccess;
+    do {
+      if (more) {
+        // tiles of step it + 1 from the positions this particle phase starts from
+        TileLists Tk = T;
+        Tk.tot = T.tot + 2 * k;
+        Tk.near_list = T.near_list + (size_t)k * T.ntiles;
+        Tk.far_list = T.far_list + (size_t)k * T.ntiles;
+        if ((he = launch_tiles(c, Tk)) != hipSuccess) break;
+        if ((he = hipEventRecord(c.ev_tiles, sB)) != hipSuccess) break;
+      }
+      if ((he = hipStreamWaitEvent(sB, c.ev_coll, 0)) != hipSuccess) break;
+      // beads_collision with beads_lubforce and the map clearing in its grid
+      if ((he = launch_bounce_back(c, true)) != hipSuccess) break;
+      c.f_version++;
+      if ((prc = mfc_streaming(ctx))) break;
+      if ((he = launch_move(c, it == c.p.irelease ? 1 : 0, 1, true)) != hipSuccess) break;
+      P.map_old = P.map_cur;
+      if ((prc = links_now(ctx, 1, it, true))) break;
+      if ((he = launch_filling(c, false, nullptr)) != hipSuccess) break;
+      c.f_version++;
+      if ((prc = mfc_macrovar(ctx))) break;
+      he = hipEventRecord(c.ev_part, sB);
+    } while (false);
+    c.stream = sA;
+    if (he != hipSuccess) return fail(ctx, MFC_ERR_HIP, "overlapped particle phase", he);
+    if (prc) return prc;
+    if (!more) {
+      HIPCHK(hipStreamWaitEvent(sA, c.ev_part, 0));
+      break;
+    }
+    // step it + 1: FORCING, then the collide of the far tiles beside the
+    // particle phase, then of the near tiles after it
+    if ((rc = mfc_forcing(ctx))) return rc;
+    if (c.phase != PHASE_POST_STREAM || c.mom != MOM_VIRTUAL || c.mom_version != c.f_version)
+      return fail(ctx, MFC_ERR_STATE, "overlapped step: moments not from the last macrovar");
+    const bool timed = c.timing && c.tcount++ % c.timing == 0 && c.tpool_used + 4 <= (int)c.tpool.size();
+    HIPCHK(hipStreamWaitEvent(sA, c.ev_tiles, 0));
+    if ((rc = collide_tiles(ctx, k, true, timed))) return rc;
+    HIPCHK(hipStreamWaitEvent(sA, c.ev_part, 0));
+    if ((rc = collide_tiles(ctx, k, false, timed))) return rc;
+    if (timed) c.ttimed++;
+    if (c.g.lay == LAY_AB)
+      c.cur ^= 1;
+    else
+      c.g.lay = c.g.lay == LAY_AA_ODD ? LAY_AA_EVEN : LAY_AA_ODD;
+    c.phase = PHASE_PRE_STREAM;
+    c.f_version++;
+  }
+  return MFC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int mfc_step(mfc_ctx* ctx, int32_t istep, int32_t nsteps) {
   NEEDCTX;
   Ctx& c = ctx->c;
   int rc;
   Particles& P = c.pt;
+  if (nsteps > 0 && c.overlap && particles_on(c) && istep >= c.p.irelease && !c.ens &&
+      P.map_np[P.map_cur] > 0 && c.phase == PHASE_POST_STREAM)
+    return step_overlapped(ctx, istep, nsteps);
   // FORCING of step s + 1 drawn during step s (spheres on, more steps in
   // this call): its table is evaluated by extra blocks of step s's refill
   // grid and made current after step s's macrovar, which keeps step s's
@@ -1164,6 +1314,7 @@ int mfc_set_timing(mfc_ctx* ctx, int32_t on) {
   HIPCHK(hipStreamSynchronize(c.stream));
   c.tpool_used = 0;
   c.tcount = 0;
+  c.ttimed = 0;
   c.timing = on > 0 ? on : 0;
   return MFC_OK;
 }
@@ -1179,7 +1330,7 @@ int mfc_get_timing(mfc_ctx* ctx, double* total_ms, int64_t* launches) {
     t += ms;
   }
   if (total_ms) *total_ms = t;
-  if (launches) *launches = c.tpool_used / 2;
+  if (launches) *launches = c.ttimed;
   return MFC_OK;
 }
 

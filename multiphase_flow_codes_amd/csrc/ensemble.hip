@@ -7,18 +7,27 @@
 // (+ map clearing), move, link scan, link offsets, link expansion (+ refill
 // search), refill.  Each grid reads its members' argument blocks from a
 // per-stage table and finds its member from a block-offset table
-// (off[c] <= blockIdx.x < off[c+1]); both tables for a step are built on the
-// host and uploaded with one copy.  The sequence and the per-member state
+// (off[c] <= blockIdx.x < off[c+1]).  The sequence and the per-member state
 // transitions are exactly those of mfc_step, so a member advanced in the
 // ensemble is bitwise identical to the same box advanced alone.
 //
+// Steps go in chunks of up to kChunk: the host builds the tables of all the
+// chunk's steps into one pinned buffer and uploads them with ONE plain copy;
+// then the chunk's stage grids (8 per step) run, in graph mode as one
+// recorded HIP graph replayed for every chunk of the same launch shapes.
+// The graphs hold kernel nodes only (their table pointers point into the
+// device table buffer); a graph is destroyed before any buffer its nodes
+// reference is freed.
+//
 // The collide grid numbers a member's nodes x fastest in blocks of BS
-// threads; BS (64, 128 or 256) is picked by mfc_ens_autotune, which times the
-// candidates on the current membership.
+// threads (the pencil), optionally dealt to the XCDs in contiguous ranges;
+// mfc_ens_autotune picks BS (64, 128, 256) and the tile order by timing
+// every pair on the current membership.
 #include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -32,6 +41,8 @@ namespace {
 
 constexpr int kStages = 8;          // collide + the seven particle stages
 constexpr int kRing = 3;            // pinned table buffers in flight
+constexpr int kChunk = 8;           // steps per table upload and per recorded graph
+constexpr int kGraphCache = 4;      // recorded chunk graphs kept (by launch shapes)
 constexpr int kFillBlocksPerBox = 32;
 constexpr int kContactBlocksPerBox = 1;   // contact pre-pass: contacts are rare
 
@@ -48,18 +59,25 @@ struct mfc_ens {
   hipStream_t stream = nullptr;
   int prec = -1, walls = -1;
   int collide_bs = 256;
-  // device copy of one step's tables, and the pinned ring they come from
+  int collide_remap = 0;    // 1: collide blocks dealt to the XCDs in contiguous ranges
+  // device copy of one chunk's tables, and the pinned ring they come from
   char* d_tab = nullptr;
   size_t d_cap = 0;
   char* h_tab[kRing] = {};
   size_t h_cap[kRing] = {};
   hipEvent_t h_ev[kRing] = {};
   int ring = 0;
-  // graph mode: each ring slot's step (table copy + the stage grids) is
-  // captured once and replayed while the grid shapes stay the same
+  // graph mode: recorded chunks, keyed by what they depend on besides the
+  // table contents (buffers, sizes, launch shapes, kernel variants)
   int use_graph = 0;
-  hipGraphExec_t gexec[kRing] = {};
-  std::vector<long long> gsig[kRing];
+  struct Rec {
+    std::vector<long long> sig;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<Rec> graphs;
+  // host time spent building and enqueueing steps (waits for the ring excluded)
+  double host_ms = 0.0;
+  long long host_steps = 0;
   std::string err;
 };
 
@@ -82,6 +100,10 @@ int efail(mfc_ens* e, int code, const char* what, hipError_t h = hipSuccess) {
   } while (0)
 
 inline size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
+
+inline double host_ms_now() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 // One stage's part of a step's tables: argument blocks and block offsets.
 struct StageTab {
@@ -111,14 +133,28 @@ int check_member(mfc_ens* e, Ctx& c) {
   return MFC_OK;
 }
 
-// Build the tables of one step for every member (advancing each member's
-// host-side state exactly as mfc_step would), upload them, launch the stages.
-int ens_one_step(mfc_ens* e) {
+// Drops every recorded graph (after the stream has drained).
+int drop_graphs(mfc_ens* e) {
+  if (e->graphs.empty()) return MFC_OK;
+  EHIP(hipStreamSynchronize(e->stream));
+  for (auto& g : e->graphs)
+    if (g.exec) EHIP(hipGraphExecDestroy(g.exec));
+  e->graphs.clear();
+  return MFC_OK;
+}
+
+// One step's tables for every member, advancing each member's host-side
+// state exactly as mfc_step would.
+struct StepTabs {
   StageTab coll, st[7];
-  coll.arg_bytes = sizeof(CollideArgs);
   bool arr_any = false;
-  const Stage order[7] = {SG_CONTACT, SG_BOUNCE, SG_MOVE, SG_SCAN, SG_COUNT, SG_EXPAND, SG_FILL};
-  for (int k = 0; k < 7; ++k) st[k].arg_bytes = stage_arg_bytes(order[k]);
+};
+const Stage kOrder[7] = {SG_CONTACT, SG_BOUNCE, SG_MOVE, SG_SCAN, SG_COUNT, SG_EXPAND, SG_FILL};
+
+int build_step(mfc_ens* e, StepTabs& T) {
+  T.coll.arg_bytes = sizeof(CollideArgs);
+  for (int k = 0; k < 7; ++k) T.st[k].arg_bytes = stage_arg_bytes(kOrder[k]);
+  StageTab* st = T.st;
   for (Member& mb : e->m) {
     Ctx& c = mb.box->c;
     Particles& P = c.pt;
@@ -131,8 +167,8 @@ int ens_one_step(mfc_ens* e) {
         return efail(e, MFC_ERR_STATE, "ensemble: member moments out of date");
       CollideArgs a = collide_args(c);
       a.arr = c.mom == MOM_ARRAYS ? 1 : 0;
-      arr_any = arr_any || a.arr;
-      coll.add(&a, (int)((c.g.ncell + e->collide_bs - 1) / e->collide_bs));
+      T.arr_any = T.arr_any || a.arr;
+      T.coll.add(&a, (int)((c.g.ncell + e->collide_bs - 1) / e->collide_bs));
       c.cur ^= 1;
       c.phase = PHASE_PRE_STREAM;
       c.f_version++;
@@ -172,25 +208,49 @@ int ens_one_step(mfc_ens* e) {
     c.ft.mac = c.ft.cur;
     mb.istep = it + 1;
   }
-  // lay the tables out in one buffer: [args][off] per stage
-  StageTab* tabs[kStages] = {&coll, &st[0], &st[1], &st[2], &st[3], &st[4], &st[5], &st[6]};
-  size_t pos[kStages][2], total = 0;
-  for (int k = 0; k < kStages; ++k) {
-    pos[k][0] = total;
-    total = align_up(total + tabs[k]->args.size());
-    pos[k][1] = total;
-    total = align_up(total + sizeof(int) * tabs[k]->off.size());
+  return MFC_OK;
+}
+
+// `n` (1..kChunk) steps of every member: tables built and uploaded at once,
+// then the stage grids of all n steps (one recorded graph in graph mode).
+int ens_chunk(mfc_ens* e, int n) {
+  const double t0 = host_ms_now();
+  StepTabs T[kChunk];
+  for (int s = 0; s < n; ++s) {
+    const int rc = build_step(e, T[s]);
+    if (rc) return rc;
   }
+  // lay the tables out in one buffer: per step, per stage [args][off]
+  size_t pos[kChunk][kStages][2], total = 0;
+  for (int s = 0; s < n; ++s) {
+    StageTab* tabs[kStages] = {&T[s].coll, &T[s].st[0], &T[s].st[1], &T[s].st[2], &T[s].st[3], &T[s].st[4],
+                               &T[s].st[5], &T[s].st[6]};
+    for (int k = 0; k < kStages; ++k) {
+      pos[s][k][0] = total;
+      total = align_up(total + tabs[k]->args.size());
+      pos[s][k][1] = total;
+      total = align_up(total + sizeof(int) * tabs[k]->off.size());
+    }
+  }
+  double waited = 0.0;
   const int r = e->ring;
   e->ring = (e->ring + 1) % kRing;
-  EHIP(hipEventSynchronize(e->h_ev[r]));
+  {
+    const double w0 = host_ms_now();
+    EHIP(hipEventSynchronize(e->h_ev[r]));   // the slot's last upload has been read
+    waited += host_ms_now() - w0;
+  }
   if (e->h_cap[r] < total) {
+    // no graph references host memory: the old buffer's copy has completed
     if (e->h_tab[r]) EHIP(hipHostFree(e->h_tab[r]));
     e->h_tab[r] = nullptr;
     EHIP(hipHostMalloc((void**)&e->h_tab[r], total * 2, hipHostMallocDefault));
     e->h_cap[r] = total * 2;
   }
   if (e->d_cap < total) {
+    // the recorded graphs point into the old device buffer: drop them first
+    const int rc = drop_graphs(e);
+    if (rc) return rc;
     EHIP(hipStreamSynchronize(e->stream));
     if (e->d_tab) EHIP(hipFree(e->d_tab));
     e->d_tab = nullptr;
@@ -198,64 +258,81 @@ int ens_one_step(mfc_ens* e) {
     e->d_cap = total * 2;
   }
   char* h = e->h_tab[r];
-  for (int k = 0; k < kStages; ++k) {
-    if (!tabs[k]->args.empty()) memcpy(h + pos[k][0], tabs[k]->args.data(), tabs[k]->args.size());
-    memcpy(h + pos[k][1], tabs[k]->off.data(), sizeof(int) * tabs[k]->off.size());
+  for (int s = 0; s < n; ++s) {
+    StageTab* tabs[kStages] = {&T[s].coll, &T[s].st[0], &T[s].st[1], &T[s].st[2], &T[s].st[3], &T[s].st[4],
+                               &T[s].st[5], &T[s].st[6]};
+    for (int k = 0; k < kStages; ++k) {
+      if (!tabs[k]->args.empty()) memcpy(h + pos[s][k][0], tabs[k]->args.data(), tabs[k]->args.size());
+      memcpy(h + pos[s][k][1], tabs[k]->off.data(), sizeof(int) * tabs[k]->off.size());
+    }
   }
   char* d = e->d_tab;
+  EHIP(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, e->stream));
   auto enqueue = [&]() -> int {
-    EHIP(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, e->stream));
-    EHIP(launch_batched_collide(e->prec, e->walls, e->collide_bs, d + pos[0][0], (const int*)(d + pos[0][1]),
-                                coll.nc, coll.off.back(), arr_any, e->stream));
-    for (int k = 0; k < 7; ++k) {
-      StageTab& t = st[k];
-      EHIP(launch_batched(order[k], e->prec, e->walls, d + pos[k + 1][0], (const int*)(d + pos[k + 1][1]),
-                          t.nc, t.off.back(), t.lds, e->stream));
+    for (int s = 0; s < n; ++s) {
+      const StepTabs& U = T[s];
+      EHIP(launch_batched_collide(e->prec, e->walls, e->collide_bs, e->collide_remap, d + pos[s][0][0],
+                                  (const int*)(d + pos[s][0][1]), U.coll.nc, U.coll.off.back(), U.arr_any,
+                                  e->stream));
+      for (int k = 0; k < 7; ++k) {
+        const StageTab& t = U.st[k];
+        EHIP(launch_batched(kOrder[k], e->prec, e->walls, d + pos[s][k + 1][0],
+                            (const int*)(d + pos[s][k + 1][1]), t.nc, t.off.back(), t.lds, e->stream));
+      }
     }
     return MFC_OK;
   };
   if (!e->use_graph) {
     const int rc = enqueue();
     if (rc) return rc;
-    EHIP(hipEventRecord(e->h_ev[r], e->stream));
-    return MFC_OK;
-  }
-  // what the captured work depends on besides the table contents: buffers,
-  // sizes, launch shapes and the collide kernel (block size, array moments)
-  std::vector<long long> sig = {(long long)(uintptr_t)d, (long long)(uintptr_t)h, (long long)total,
-                                e->collide_bs, e->prec, e->walls, coll.nc, coll.off.back(),
-                                arr_any ? 1LL : 0LL};
-  for (int k = 0; k < 7; ++k) {
-    sig.push_back(st[k].nc);
-    sig.push_back(st[k].off.back());
-    sig.push_back((long long)st[k].lds);
-  }
-  for (int k = 0; k < kStages; ++k) {
-    sig.push_back((long long)pos[k][0]);
-    sig.push_back((long long)pos[k][1]);
-  }
-  if (!e->gexec[r] || e->gsig[r] != sig) {
-    if (e->gexec[r]) EHIP(hipGraphExecDestroy(e->gexec[r]));
-    e->gexec[r] = nullptr;
-    hipGraph_t g = nullptr;
-    EHIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = enqueue();
-    const hipError_t end = hipStreamEndCapture(e->stream, &g);
-    if (rc) {
-      if (g) hipGraphDestroy(g);
-      return rc;
+  } else {
+    std::vector<long long> sig = {(long long)(uintptr_t)d, (long long)total, n, e->collide_bs,
+                                  e->collide_remap, e->prec, e->walls};
+    for (int s = 0; s < n; ++s) {
+      sig.push_back(T[s].coll.nc);
+      sig.push_back(T[s].coll.off.back());
+      sig.push_back(T[s].arr_any ? 1LL : 0LL);
+      for (int k = 0; k < 7; ++k) {
+        sig.push_back(T[s].st[k].nc);
+        sig.push_back(T[s].st[k].off.back());
+        sig.push_back((long long)T[s].st[k].lds);
+      }
+      for (int k = 0; k < kStages; ++k) {
+        sig.push_back((long long)pos[s][k][0]);
+        sig.push_back((long long)pos[s][k][1]);
+      }
     }
-    if (end != hipSuccess) return efail(e, MFC_ERR_HIP, "hipStreamEndCapture", end);
-    const hipError_t inst = hipGraphInstantiate(&e->gexec[r], g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
-    if (inst != hipSuccess) {
-      e->gexec[r] = nullptr;
-      return efail(e, MFC_ERR_HIP, "hipGraphInstantiate", inst);
+    mfc_ens::Rec* rec = nullptr;
+    for (auto& g : e->graphs)
+      if (g.sig == sig) rec = &g;
+    if (!rec) {
+      if ((int)e->graphs.size() >= kGraphCache) {
+        // the oldest recording may still be queued: let the stream drain
+        EHIP(hipStreamSynchronize(e->stream));
+        EHIP(hipGraphExecDestroy(e->graphs.front().exec));
+        e->graphs.erase(e->graphs.begin());
+      }
+      hipGraph_t g = nullptr;
+      EHIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue();
+      const hipError_t end = hipStreamEndCapture(e->stream, &g);
+      if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+      }
+      if (end != hipSuccess) return efail(e, MFC_ERR_HIP, "hipStreamEndCapture", end);
+      hipGraphExec_t ex = nullptr;
+      const hipError_t inst = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (inst != hipSuccess) return efail(e, MFC_ERR_HIP, "hipGraphInstantiate", inst);
+      e->graphs.push_back({sig, ex});
+      rec = &e->graphs.back();
     }
-    e->gsig[r] = sig;
+    EHIP(hipGraphLaunch(rec->exec, e->stream));
   }
-  EHIP(hipGraphLaunch(e->gexec[r], e->stream));
   EHIP(hipEventRecord(e->h_ev[r], e->stream));
+  e->host_ms += host_ms_now() - t0 - waited;
+  e->host_steps += n;
   return MFC_OK;
 }
 
@@ -292,12 +369,14 @@ int mfc_ens_destroy(mfc_ens* e) {
     mb.box->c.ens = nullptr;
   }
   e->m.clear();
+  // graphs first: their nodes point into the table buffers freed below
+  for (auto& g : e->graphs)
+    if (g.exec) hipGraphExecDestroy(g.exec);
+  e->graphs.clear();
   for (int r = 0; r < kRing; ++r) {
     if (e->h_tab[r]) hipHostFree(e->h_tab[r]);
     if (e->h_ev[r]) hipEventDestroy(e->h_ev[r]);
   }
-  for (int r = 0; r < kRing; ++r)
-    if (e->gexec[r]) hipGraphExecDestroy(e->gexec[r]);
   if (e->d_tab) hipFree(e->d_tab);
   if (e->stream) hipStreamDestroy(e->stream);
   delete e;
@@ -348,10 +427,19 @@ int mfc_ens_size(const mfc_ens* e) { return e ? (int)e->m.size() : -1; }
 int mfc_ens_step(mfc_ens* e, int32_t nsteps) {
   if (!e || nsteps < 0) return MFC_ERR_ARG;
   if (e->m.empty()) return MFC_OK;
-  for (int s = 0; s < nsteps; ++s) {
-    int rc = ens_one_step(e);
+  for (int s = 0; s < nsteps; s += kChunk) {
+    int rc = ens_chunk(e, std::min(kChunk, nsteps - s));
     if (rc) return rc;
   }
+  return MFC_OK;
+}
+
+int mfc_ens_host_time(mfc_ens* e, double* ms, int64_t* steps) {
+  if (!e) return MFC_ERR_ARG;
+  if (ms) *ms = e->host_ms;
+  if (steps) *steps = e->host_steps;
+  e->host_ms = 0.0;
+  e->host_steps = 0;
   return MFC_OK;
 }
 
@@ -378,36 +466,52 @@ int mfc_ens_set_block(mfc_ens* e, int32_t bs) {
   return MFC_OK;
 }
 
-// Time `steps` ensemble steps with each collide block size and keep the
-// fastest.  The members advance by 3 * steps steps.
+int mfc_ens_set_tile_order(mfc_ens* e, int32_t order) {
+  if (!e || (order != 0 && order != 1)) return MFC_ERR_ARG;
+  e->collide_remap = order;
+  return MFC_OK;
+}
+
+// Time `steps` ensemble steps with each pair (collide block size, tile
+// order), after `steps` untimed steps that record the pair's graphs, and
+// keep the fastest.  The members advance by 12 * steps steps.
 int mfc_ens_autotune(mfc_ens* e, int32_t steps, int32_t* chosen, double* ms_per_step) {
   if (!e || steps < 1) return MFC_ERR_ARG;
   if (e->m.empty()) return efail(e, MFC_ERR_STATE, "autotune of an empty ensemble");
-  const int cand[3] = {64, 128, 256};
+  const int bsz[3] = {64, 128, 256};
   hipEvent_t t0, t1;
   EHIP(hipEventCreate(&t0));
   EHIP(hipEventCreate(&t1));
-  int best = 256;
+  int best_bs = 256, best_order = 0;
   float best_ms = 1e30f;
-  for (int k = 0; k < 3; ++k) {
-    e->collide_bs = cand[k];
-    EHIP(hipEventRecord(t0, e->stream));
+  for (int k = 0; k < 6; ++k) {
+    e->collide_bs = bsz[k % 3];
+    e->collide_remap = k / 3;
     int rc = mfc_ens_step(e, steps);
+    if (rc) return rc;
+    EHIP(hipEventRecord(t0, e->stream));
+    rc = mfc_ens_step(e, steps);
     if (rc) return rc;
     EHIP(hipEventRecord(t1, e->stream));
     EHIP(hipEventSynchronize(t1));
     float ms = 0.f;
     EHIP(hipEventElapsedTime(&ms, t0, t1));
-    if (ms_per_step) ms_per_step[k] = ms / steps;
+    ms /= (float)steps;
+    if (ms_per_step) ms_per_step[k] = ms;
     if (ms < best_ms) {
       best_ms = ms;
-      best = cand[k];
+      best_bs = bsz[k % 3];
+      best_order = k / 3;
     }
   }
   hipEventDestroy(t0);
   hipEventDestroy(t1);
-  e->collide_bs = best;
-  if (chosen) *chosen = best;
+  e->collide_bs = best_bs;
+  e->collide_remap = best_order;
+  if (chosen) {
+    chosen[0] = best_bs;
+    chosen[1] = best_order;
+  }
   return MFC_OK;
 }
 

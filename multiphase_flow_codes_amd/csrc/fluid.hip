@@ -234,6 +234,31 @@ __global__ void __launch_bounds__(256)
   collide_node<T, R, ARR, WALLS, LAY>(a, x, (int)y, (int)z);
 }
 
+// The collide of the tiles of a list (overlapped step): each wavefront takes
+// one tile (kTileX nodes of a row) at a time.  The grid is a fixed number of
+// workgroups; the list, whose length is a device word, is split into eight
+// contiguous ranges, one per XCD (workgroup b runs on XCD b % 8), and the
+// wavefronts of an XCD sweep its range in order, so neighbouring rows meet
+// in one L2.  Every wavefront leaves the loop when the range is done.
+template <typename T, typename R, bool WALLS, int LAY>
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && sizeof(R) == 8 && LAY == LAY_AB ? 4 : 1)))
+    k_collide_tiles(CollideArgs a, const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
+                    uint32_t tpr) {
+  const uint32_t n = *count;
+  const uint32_t xcd = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+  const uint32_t waves = (gridDim.x / kXcds) * (blockDim.x >> 6);   // wavefronts per XCD
+  const uint32_t lo = (uint32_t)((uint64_t)n * xcd / kXcds), hi = (uint32_t)((uint64_t)n * (xcd + 1) / kXcds);
+  const uint32_t lane = threadIdx.x & 63u;
+  for (uint32_t L = lo + j * (blockDim.x >> 6) + (threadIdx.x >> 6); L < hi; L += waves) {
+    const uint32_t t = list[L];
+    const uint32_t row = t / tpr;
+    const int x = (int)((t - row * tpr) * (uint32_t)kTileX + lane);
+    if (x < a.g.nx)
+      collide_node<T, R, false, WALLS, LAY>(a, x, (int)(row % (uint32_t)a.g.ny), (int)(row / (uint32_t)a.g.ny));
+  }
+}
+
 // Several boxes in one grid (ensemble engine): block b belongs to box c with
 // off[c] <= b < off[c+1]; the box's nodes are numbered x fastest and taken
 // BS per block, so consecutive lanes still read consecutive x (and rows
@@ -249,12 +274,22 @@ __device__ __forceinline__ int case_of_block(const int* __restrict__ off, int nc
 
 // ARR_ANY: some member takes its moments from the rho/u arrays (a.arr, its
 // first step after joining); without it only the recomputing path is built.
+// remap = 1 deals the grid to the 8 XCDs in contiguous ranges (workgroups are
+// dispatched to the XCDs round-robin, so launch-order block b runs on XCD
+// b % 8): consecutive node ranges, whose neighbour rows overlap, then share
+// one L2.  The block-to-nodes map is a bijection either way, so results do
+// not depend on it.
 template <typename T, typename R, bool WALLS, int BS, bool ARR_ANY>
 __global__ void __launch_bounds__(BS)
-    kb_collide(const CollideArgs* __restrict__ t, const int* __restrict__ off, int nc) {
-  const int c = case_of_block(off, nc, (int)blockIdx.x);
+    kb_collide(const CollideArgs* __restrict__ t, const int* __restrict__ off, int nc, int remap) {
+  uint32_t b = blockIdx.x;
+  if (remap) {
+    const uint32_t per = gridDim.x / kXcds;
+    if (b < per * kXcds) b = (b % kXcds) * per + b / kXcds;
+  }
+  const int c = case_of_block(off, nc, (int)b);
   const CollideArgs a = t[c];
-  const uint64_t n = (uint64_t)((int)blockIdx.x - off[c]) * BS + threadIdx.x;
+  const uint64_t n = (uint64_t)((int)b - off[c]) * BS + threadIdx.x;
   if (n >= a.g.ncell) return;
   const uint32_t nx = (uint32_t)a.g.nx, ny = (uint32_t)a.g.ny;
   const uint32_t x = (uint32_t)(n % nx);
@@ -509,6 +544,42 @@ static void collide_dispatch(Ctx& c, bool arrays, const CollideArgs& a) {
     collide_dispatch_lay<T, R, LAY_AB>(c, arrays, a);
 }
 
+// grid of the tile collide: 8 XCDs x 256 CUs' worth of 256-thread workgroups
+constexpr unsigned kTileGrid = 8 * 1024;
+
+template <typename T, typename R, int LAY>
+static void collide_tiles_lay(Ctx& c, const CollideArgs& a, const uint32_t* list, const uint32_t* count) {
+  const uint32_t tpr = (uint32_t)((c.g.nx + kTileX - 1) / kTileX);
+  if (c.g.walls_y)
+    hipLaunchKernelGGL((k_collide_tiles<T, R, true, LAY>), dim3(kTileGrid), dim3(256), 0, c.stream, a, list,
+                       count, tpr);
+  else
+    hipLaunchKernelGGL((k_collide_tiles<T, R, false, LAY>), dim3(kTileGrid), dim3(256), 0, c.stream, a, list,
+                       count, tpr);
+}
+
+template <typename T, typename R>
+static void collide_tiles_dispatch(Ctx& c, const CollideArgs& a, const uint32_t* list, const uint32_t* count) {
+  if (c.g.lay == LAY_AA_ODD)
+    collide_tiles_lay<T, R, LAY_AA_ODD>(c, a, list, count);
+  else if (c.g.lay == LAY_AA_EVEN)
+    collide_tiles_lay<T, R, LAY_AA_EVEN>(c, a, list, count);
+  else
+    collide_tiles_lay<T, R, LAY_AB>(c, a, list, count);
+}
+
+hipError_t launch_collide_tiles(Ctx& c, const uint32_t* list, const uint32_t* count, bool far) {
+  CollideArgs a = collide_args(c);
+  if (far) a.mask = nullptr;   // no node of a far tile is solid in any map of the step
+  if (c.prec == MFC_PREC_FP64)
+    collide_tiles_dispatch<double, double>(c, a, list, count);
+  else if (c.prec == MFC_PREC_FP32_F64MATH)
+    collide_tiles_dispatch<float, double>(c, a, list, count);
+  else
+    collide_tiles_dispatch<float, float>(c, a, list, count);
+  return hipGetLastError();
+}
+
 hipError_t launch_collide(Ctx& c, bool arrays) {
   const CollideArgs a = collide_args(c);
   if (c.prec == MFC_PREC_FP64)
@@ -643,40 +714,42 @@ namespace mfc {
 
 template <typename T, typename R, int BS, bool ARR_ANY>
 static void batched_collide_k(int walls, const CollideArgs* tab, const int* off, int nc, int blocks,
-                              hipStream_t s) {
+                              int remap, hipStream_t s) {
   if (walls)
-    hipLaunchKernelGGL((kb_collide<T, R, true, BS, ARR_ANY>), dim3(blocks), dim3(BS), 0, s, tab, off, nc);
+    hipLaunchKernelGGL((kb_collide<T, R, true, BS, ARR_ANY>), dim3(blocks), dim3(BS), 0, s, tab, off, nc,
+                       remap);
   else
-    hipLaunchKernelGGL((kb_collide<T, R, false, BS, ARR_ANY>), dim3(blocks), dim3(BS), 0, s, tab, off, nc);
+    hipLaunchKernelGGL((kb_collide<T, R, false, BS, ARR_ANY>), dim3(blocks), dim3(BS), 0, s, tab, off, nc,
+                       remap);
 }
 
 template <int BS, bool ARR_ANY>
 static void batched_collide_bs(int prec, int walls, const void* t, const int* off, int nc, int blocks,
-                               hipStream_t s) {
+                               int remap, hipStream_t s) {
   const CollideArgs* tab = (const CollideArgs*)t;
   if (prec == MFC_PREC_FP64)
-    batched_collide_k<double, double, BS, ARR_ANY>(walls, tab, off, nc, blocks, s);
+    batched_collide_k<double, double, BS, ARR_ANY>(walls, tab, off, nc, blocks, remap, s);
   else if (prec == MFC_PREC_FP32_F64MATH)
-    batched_collide_k<float, double, BS, ARR_ANY>(walls, tab, off, nc, blocks, s);
+    batched_collide_k<float, double, BS, ARR_ANY>(walls, tab, off, nc, blocks, remap, s);
   else
-    batched_collide_k<float, float, BS, ARR_ANY>(walls, tab, off, nc, blocks, s);
+    batched_collide_k<float, float, BS, ARR_ANY>(walls, tab, off, nc, blocks, remap, s);
 }
 
-hipError_t launch_batched_collide(int prec, int walls, int bs, const void* t, const int* off, int nc,
-                                  int blocks, bool arr_any, hipStream_t s) {
+hipError_t launch_batched_collide(int prec, int walls, int bs, int remap, const void* t, const int* off,
+                                  int nc, int blocks, bool arr_any, hipStream_t s) {
   if (nc <= 0 || blocks <= 0) return hipSuccess;
   switch (bs) {
     case 64:
-      arr_any ? batched_collide_bs<64, true>(prec, walls, t, off, nc, blocks, s)
-              : batched_collide_bs<64, false>(prec, walls, t, off, nc, blocks, s);
+      arr_any ? batched_collide_bs<64, true>(prec, walls, t, off, nc, blocks, remap, s)
+              : batched_collide_bs<64, false>(prec, walls, t, off, nc, blocks, remap, s);
       break;
     case 128:
-      arr_any ? batched_collide_bs<128, true>(prec, walls, t, off, nc, blocks, s)
-              : batched_collide_bs<128, false>(prec, walls, t, off, nc, blocks, s);
+      arr_any ? batched_collide_bs<128, true>(prec, walls, t, off, nc, blocks, remap, s)
+              : batched_collide_bs<128, false>(prec, walls, t, off, nc, blocks, remap, s);
       break;
     case 256:
-      arr_any ? batched_collide_bs<256, true>(prec, walls, t, off, nc, blocks, s)
-              : batched_collide_bs<256, false>(prec, walls, t, off, nc, blocks, s);
+      arr_any ? batched_collide_bs<256, true>(prec, walls, t, off, nc, blocks, remap, s)
+              : batched_collide_bs<256, false>(prec, walls, t, off, nc, blocks, remap, s);
       break;
     default: return hipErrorInvalidValue;
   }

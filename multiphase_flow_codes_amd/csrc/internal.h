@@ -29,6 +29,7 @@ enum StatusWord {
   ST_NSOLID = 8,         // solid nodes marked by the last beads_links
   ST_BOX_FAULT = 9,      // particle boxes that did not fit the lattice
   ST_CELL_OVERFLOW = 10, // pair-search cells or neighbour lists over capacity
+  ST_FAST = 11,          // sphere steps that moved a centre a node or more
   ST_COUNT = 16
 };
 
@@ -43,6 +44,25 @@ constexpr int kScanChunk = 4096;
 // a segment's links are split over up to this many bounce-back workgroups
 constexpr int kMaxBBSplit = 4;
 constexpr int kCellCap = 16;
+// overlapped step: tiles of kTileX consecutive x nodes of a row, classified
+// near/far in chunks of kTileChunk tiles
+constexpr int kTileX = 64;
+constexpr int kTileChunk = 4096;
+
+// Near and far tile lists of the overlapped step (device), rebuilt every step
+// from the sphere positions at the start of its particle phase.
+struct TileLists {
+  int tpr = 0;                 // tiles per row
+  uint64_t ntiles = 0;
+  double reach = 0.0;          // a tile within this distance of a centre is near
+  uint8_t* mark = nullptr;     // per tile: 1 near (cleared again by the compaction)
+  int32_t* cnt = nullptr;      // near tiles per chunk
+  uint32_t* off_near = nullptr;
+  uint32_t* off_far = nullptr;
+  uint32_t* tot = nullptr;     // [0] near tiles, [1] far tiles
+  uint32_t* near_list = nullptr;
+  uint32_t* far_list = nullptr;
+};
 // The live forcing modes of one FORCING call, grouped by kx (n[0] modes with
 // kx = 0 first, then kx = 1, then kx = 2), each in (kz, ky) order, with
 // (re, im) of the three components.  Handed to the table kernel by value.
@@ -135,6 +155,7 @@ struct Ctx {
   double *rho = nullptr, *ux = nullptr, *uy = nullptr, *uz = nullptr;
   int mom = MOM_NONE;
   uint64_t mom_version = 0;
+  uint64_t arrays_version = 0;      // MOM_VIRTUAL: the macrovar whose rho/u the arrays hold (0: none)
   ForceTables ft;
   // host forcing state (gaussian.f90 FORCING)
   int32_t iseedf = 0, iyf = 0, ivf[32] = {0};
@@ -151,6 +172,8 @@ struct Ctx {
   // `timing`-th launch (0: off), summed by mfc_get_timing
   int timing = 0;
   int64_t tcount = 0;
+  int64_t ttimed = 0;               // collides timed (an overlapped collide is two launches)
+  bool overlap_used = false;        // an overlapped step ran (its displacement bound is checked)
   std::vector<hipEvent_t> tpool;
   int tpool_used = 0;
   // diagnostics scratch, allocated on first use
@@ -161,11 +184,21 @@ struct Ctx {
   double2* diag_tw[3] = {nullptr, nullptr, nullptr};   // exp(-2 pi i j / n) per axis
   void* ens = nullptr;              // the ensemble this box is a member of (its stream is on loan)
   int xcd_remap = -1;               // collide tile order over the XCDs (-1: by shape; MFC_XCD_REMAP=0/1/2)
+  // overlapped step (mfc_step with spheres): the particle phase of step n runs
+  // on pstream while the collide of step n + 1 covers the tiles far from every
+  // sphere; MFC_OVERLAP=0 turns it off
+  int overlap = 1;
+  hipStream_t pstream = nullptr;
+  hipEvent_t ev_coll = nullptr, ev_tiles = nullptr, ev_part = nullptr;
+  TileLists tiles;
 };
 
 // ---- launchers (fluid.hip) ----
 hipError_t launch_force_table(Ctx& c, int slot, const ModeSet& m);
 hipError_t launch_collide(Ctx& c, bool arrays);
+// the collide (moments recomputed, never the arrays) of the tiles in `list`
+// (*count of them, a device word); with far set the solid mask is not read
+hipError_t launch_collide_tiles(Ctx& c, const uint32_t* list, const uint32_t* count, bool far);
 hipError_t launch_macro(Ctx& c, bool pulled);
 hipError_t launch_rhoupdat(Ctx& c, bool pulled, double* d_maxdiff);
 hipError_t launch_initpop(Ctx& c);
@@ -195,6 +228,8 @@ hipError_t launch_initpartvel(Ctx& c);
 // (re)build the pair-search cells if they are on and stale
 hipError_t ensure_cells(Ctx& c);
 hipError_t launch_maps_to_host_layout(Ctx& c, int which, int32_t* ib, int32_t* is);
+// near/far tile lists from the current positions (c.tiles allocated)
+hipError_t launch_tiles(Ctx& c, TileLists& T);
 hipError_t launch_links_dense(Ctx& c, int32_t* ilink, int32_t* mlink, double* alink);
 
 // ---- launchers (initial.hip) ----

@@ -1030,8 +1030,15 @@ __device__ __forceinline__ void move_body(int b_, MoveArgs m, int np, int first,
     if (first) dwdt[c] = dwn[c];
     wp[c] = wpp[c] + 0.5 * dt * (dwdt[c] + dwn[c]);
   }
+  bool fast = false;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) yp[c] = yp[c] + 0.5 * dt * (wp[c] + wpp[c]);
+  for (int c = 0; c < 3; ++c) {
+    const double d = 0.5 * dt * (wp[c] + wpp[c]);
+    yp[c] = yp[c] + d;
+    fast = fast || !(fabs(d) < 1.0);
+  }
+  // the overlapped step assumes a centre moves less than one node per step
+  if (fast) atomicAdd(&status[ST_FAST], 1);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     if (first) tpp[c] = torq[c];
@@ -1677,6 +1684,162 @@ bool fill_apply_args(Ctx& c, int nblk, FillApplyArgs& a) {
   if (!make_fill_args(c, P.map_old, P.map_cur, a.fa)) return false;
   a.A = c.pop[c.cur]; a.g = c.g; a.nblk = nblk;
   return true;
+}
+
+// ---- far and near tiles of the overlapped step ----------------------------------
+// A tile is kTileX consecutive x nodes of one row (tile t = tx + tpr * row).
+// The particle phase of step n (bounce-back, lubrication, move, links,
+// refill, with the clearing of the map built two steps back) reads and
+// writes no node farther than r + 3 sqrt 2 + 2 from the centres it starts
+// from, as long as no sphere moves a node or more in one step (k_move counts
+// such steps); the next collide touches only nodes within sqrt 2 of its
+// own.  Tiles with no node within reach = r + 3 sqrt 2 + 2 + sqrt 2 of a
+// centre (periodic images; none across the y walls) are "far": their collide
+// for step n + 1 can run while the particle phase of step n does.
+
+// One workgroup per sphere marks the tiles of every row whose node centres
+// come within `reach` of the centre.
+__global__ void __launch_bounds__(PB)
+    k_tile_mark(const double* __restrict__ ypglb, int np, PGeom G, double reach, int tpr,
+                uint8_t* __restrict__ mark) {
+  const int p = blockIdx.x;
+  if (p >= np) return;
+  const double cx = ypglb[3 * p], cy = ypglb[3 * p + 1], cz = ypglb[3 * p + 2];
+  const int j0 = (int)floor(cy - reach - 0.5), j1 = (int)ceil(cy + reach - 0.5);
+  const int k0 = (int)floor(cz - reach - 0.5), k1 = (int)ceil(cz + reach - 0.5);
+  const int nj = j1 - j0 + 1, nk = k1 - k0 + 1;
+  const double r2 = reach * reach;
+  for (int t = threadIdx.x; t < nj * nk; t += PB) {
+    const int jj = j0 + t % nj, kk = k0 + t / nj;
+    const double dy = (double)jj + 0.5 - cy, dz = (double)kk + 0.5 - cz;
+    const double d2 = dy * dy + dz * dz;
+    if (d2 > r2) continue;
+    if (G.walls_y && (jj < 0 || jj >= G.ny)) continue;
+    const int j = ((jj % G.ny) + G.ny) % G.ny, k = ((kk % G.nz) + G.nz) % G.nz;
+    uint8_t* row = mark + (size_t)tpr * ((size_t)j + (size_t)G.ny * k);
+    const double e = sqrt(r2 - d2);
+    const int i0 = (int)ceil(cx - e - 0.5), i1 = (int)floor(cx + e - 0.5);
+    if (i1 - i0 + 1 >= G.nx) {
+      for (int b = 0; b < tpr; ++b) row[b] = 1;
+      continue;
+    }
+    for (int i = i0; i <= i1;) {
+      const int iw = ((i % G.nx) + G.nx) % G.nx;
+      const int b = iw / kTileX;
+      row[b] = 1;
+      // first node past this tile (the last tile of a row may be short)
+      const int end = kTileX * (b + 1) < G.nx ? kTileX * (b + 1) : G.nx;
+      i += end - iw;
+    }
+  }
+}
+
+// Near tiles per chunk of kTileChunk tiles.
+__global__ void __launch_bounds__(PB)
+    k_tile_count(const uint8_t* __restrict__ mark, uint64_t ntiles, int32_t* __restrict__ cnt) {
+  __shared__ int red[PB / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * kTileChunk;
+  int c = 0;
+  for (int i = threadIdx.x; i < kTileChunk; i += PB) {
+    const uint64_t t = base + (uint64_t)i;
+    if (t < ntiles && mark[t]) ++c;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int w = 0; w < PB / 64; ++w) s += red[w];
+    cnt[blockIdx.x] = s;
+  }
+}
+
+// Exclusive offsets of the near and far tiles of each chunk; totals in
+// tot[0] (near) and tot[1] (far).  One workgroup.
+__global__ void __launch_bounds__(1024)
+    k_tile_offsets(const int32_t* __restrict__ cnt, int nchunk, uint64_t ntiles,
+                   uint32_t* __restrict__ off_near, uint32_t* __restrict__ off_far,
+                   uint32_t* __restrict__ tot) {
+  __shared__ uint32_t part[1024];
+  const int per = (nchunk + 1023) / 1024;
+  const int c0 = threadIdx.x * per, c1 = min(nchunk, c0 + per);
+  uint32_t s = 0;
+  for (int c = c0; c < c1; ++c) s += (uint32_t)cnt[c];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const uint32_t v = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t near = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+  for (int c = c0; c < c1; ++c) {
+    const uint64_t first = (uint64_t)c * kTileChunk;
+    off_near[c] = near;
+    off_far[c] = (uint32_t)(first - near);
+    near += (uint32_t)cnt[c];
+  }
+  if (threadIdx.x == 1023) {
+    tot[0] = part[1023];
+    tot[1] = (uint32_t)(ntiles - part[1023]);
+  }
+}
+
+// Each chunk writes its tiles, in order, into the near and far lists, and
+// clears its marks for the next step.
+__global__ void __launch_bounds__(PB)
+    k_tile_emit(uint8_t* __restrict__ mark, uint64_t ntiles, const uint32_t* __restrict__ off_near,
+                const uint32_t* __restrict__ off_far, uint32_t* __restrict__ near_list,
+                uint32_t* __restrict__ far_list) {
+  __shared__ uint32_t wsum[PB / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * kTileChunk;
+  uint32_t near = off_near[blockIdx.x], far = off_far[blockIdx.x];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i0 = 0; i0 < kTileChunk; i0 += PB) {
+    const uint64_t t = base + (uint64_t)(i0 + threadIdx.x);
+    const bool in = t < ntiles;
+    const bool nr = in && mark[t] != 0;
+    const uint64_t bal = __ballot(nr);
+    const uint64_t bin = __ballot(in);
+    const uint32_t below = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+    const uint32_t below_in = (uint32_t)__popcll(bin & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[w] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+    for (int q = 0; q < PB / 64; ++q) {
+      if (q < w) pre += wsum[q];
+      tot += wsum[q];
+    }
+    // tiles of earlier waves in this pass: 64 each (all in range until the end)
+    const uint32_t pre_in = (uint32_t)(64 * w);
+    if (in) {
+      if (nr) {
+        near_list[near + pre + below] = (uint32_t)t;
+        mark[t] = 0;
+      } else {
+        far_list[far + (pre_in - pre) + (below_in - below)] = (uint32_t)t;
+      }
+    }
+    near += tot;
+    far += (uint32_t)PB - tot;
+    __syncthreads();
+  }
+}
+
+hipError_t launch_tiles(Ctx& c, TileLists& T) {
+  Particles& P = c.pt;
+  const PGeom G = pgeom(c);
+  hipLaunchKernelGGL(k_tile_mark, dim3(P.np), dim3(PB), 0, c.stream, P.ypglb, P.np, G, T.reach, T.tpr,
+                     T.mark);
+  const int nchunk = (int)((T.ntiles + kTileChunk - 1) / kTileChunk);
+  hipLaunchKernelGGL(k_tile_count, dim3(nchunk), dim3(PB), 0, c.stream, T.mark, T.ntiles, T.cnt);
+  hipLaunchKernelGGL(k_tile_offsets, dim3(1), dim3(1024), 0, c.stream, T.cnt, nchunk, T.ntiles, T.off_near,
+                     T.off_far, T.tot);
+  hipLaunchKernelGGL(k_tile_emit, dim3(nchunk), dim3(PB), 0, c.stream, T.mark, T.ntiles, T.off_near,
+                     T.off_far, T.near_list, T.far_list);
+  return hipGetLastError();
 }
 
 // ---- single-box launchers -------------------------------------------------------

@@ -199,8 +199,8 @@ enum Stage { SG_CONTACT, SG_BOUNCE, SG_MOVE, SG_SCAN, SG_COUNT, SG_EXPAND, SG_FI
 hipError_t launch_batched(Stage st, int prec, int walls, const void* t, const int* off, int nc,
                           int blocks, size_t lds, hipStream_t s);
 // arr_any: some member's CollideArgs::arr is set
-hipError_t launch_batched_collide(int prec, int walls, int bs, const void* t, const int* off, int nc,
-                                  int blocks, bool arr_any, hipStream_t s);
+hipError_t launch_batched_collide(int prec, int walls, int bs, int remap, const void* t, const int* off,
+                                  int nc, int blocks, bool arr_any, hipStream_t s);
 size_t stage_arg_bytes(Stage st);
 
 }  // namespace mfc

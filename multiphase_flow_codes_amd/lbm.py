@@ -113,7 +113,9 @@ SIGNATURES = {
     "mfc_ens_sync": [C.c_void_p],
     "mfc_ens_set_block": [C.c_void_p, C.c_int32],
     "mfc_ens_set_graph": [C.c_void_p, C.c_int32],
+    "mfc_ens_set_tile_order": [C.c_void_p, C.c_int32],
     "mfc_ens_autotune": [C.c_void_p, C.c_int32, _I, _D],
+    "mfc_ens_host_time": [C.c_void_p, _D, C.POINTER(C.c_int64)],
     "mfc_ens_get_stream": [C.c_void_p],
 }
 
@@ -405,13 +407,27 @@ class Ensemble:
         """Replay each step as a recorded HIP graph (see mfc_ens_set_graph)."""
         self._chk(lib().mfc_ens_set_graph(self.h, int(bool(on))))
 
+    def set_tile_order(self, order: int):
+        """Collide tile order: 0 launch order, 1 contiguous ranges per XCD."""
+        self._chk(lib().mfc_ens_set_tile_order(self.h, int(order)))
+
     def autotune(self, steps: int = 5):
-        """Times collide blocks of 64, 128 and 256 threads on the current
-        members (which advance 3 * steps steps); keeps the fastest."""
-        chosen = C.c_int32()
-        ms = np.zeros(3)
-        self._chk(lib().mfc_ens_autotune(self.h, int(steps), C.byref(chosen), _dp(ms)))
-        return chosen.value, dict(zip((64, 128, 256), ms))
+        """Times every (collide block of 64, 128, 256 threads, tile order 0/1)
+        pair on the current members (which advance 12 * steps steps); keeps
+        the fastest.  Returns ((block, order), {(block, order): ms/step})."""
+        chosen = np.zeros(2, dtype=np.int32)
+        ms = np.zeros(6)
+        self._chk(lib().mfc_ens_autotune(self.h, int(steps), _ip(chosen), _dp(ms)))
+        keys = [(b, o) for o in (0, 1) for b in (64, 128, 256)]
+        return (int(chosen[0]), int(chosen[1])), dict(zip(keys, ms))
+
+    def host_time(self):
+        """(host ms spent building and enqueueing steps, steps) since the
+        last call."""
+        ms = np.zeros(1)
+        n = C.c_int64()
+        self._chk(lib().mfc_ens_host_time(self.h, _dp(ms), C.byref(n)))
+        return float(ms[0]), int(n.value)
 
     def close(self):
         if self.h:

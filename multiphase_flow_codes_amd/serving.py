@@ -32,6 +32,23 @@ from .lbm import Ensemble, Lattice
 from .params import LBMParams
 
 
+def device_free_bytes() -> int:
+    """Free device memory of the current GPU (hipMemGetInfo)."""
+    import ctypes as C
+    import os
+    global _hip
+    if _hip is None:
+        root = os.environ.get("ROCM_PATH", "/opt/rocm")
+        _hip = C.CDLL(os.path.join(root, "lib", "libamdhip64.so"))
+    free, total = C.c_size_t(), C.c_size_t()
+    if _hip.hipMemGetInfo(C.byref(free), C.byref(total)) != 0:
+        raise RuntimeError("hipMemGetInfo failed")
+    return int(free.value)
+
+
+_hip = None
+
+
 @dataclass
 class BoxCase:
     n: int                      # edge length (cubic box)
@@ -59,17 +76,44 @@ class BoxCase:
     def cells(self) -> int:
         return self.n ** 3
 
+    @property
+    def terminal_drag(self) -> float:
+        """Magnitude of the mean drag at terminal velocity: the buoyant weight
+        per sphere, V_p (rho_p/rho_f - 1) g."""
+        return 4.0 / 3.0 * math.pi * self.rad ** 3 * (self.rho_ratio - 1.0) * self.body_force
+
     def footprint_bytes(self, precision: str) -> int:
-        """Device memory of the case (populations A/B, moments, maps, links)."""
+        """Device memory make_lattice allocates for the case: the sizes
+        mfc_create and mfc_prt_init request (populations A/B with the plane
+        pad, forcing tables, moments, particle arrays, segment lists, link
+        pool, refill list, node maps), each rounded up to the allocator's
+        granularity (4 KiB below 1 MiB, 2 MiB above), plus 16 MiB for the
+        runtime's per-context state.  An upper bound of the hipMemGetInfo
+        difference around make_lattice (checked by the serving test)."""
         t = 8 if precision == "fp64" else 4
-        c = self.cells
-        b = 2 * 19 * t * c + 4 * 8 * c + 2 * 4 * c + c // 4
-        r = self.rad
-        seg = (math.ceil(2 * (r + 2) + 1) + 4)
-        nseg = (seg + 3) // 4
-        b += self.npart * nseg * 4 * seg * seg * 16
-        b += self.npart * int(4 / 3 * math.pi * (r + 3) ** 3) * 9 * 21
-        return int(b)
+        n, np_, r = self.n, self.npart, self.rad
+        ncell = n ** 3
+        pitch = ncell + 512 // t
+        allocs = [19 * pitch * t] * 2 + [8 * 18 * n * n] * 4 + [8 * 6 * n, 64, 64] + [8 * ncell] * 4
+        if np_ > 0:
+            allocs += [8 * 3 * np_] * 16
+            maxcnt = math.ceil(2.0 * (r + 2.0) + 1.0) + 4
+            nseg, seg_cap = (maxcnt + 3) // 4, 4 * maxcnt * maxcnt
+            nsegs = np_ * nseg
+            nodes = nsegs * seg_cap
+            allocs += [4 * nsegs, 8 * (nsegs + 1), 16 * (nsegs // 4096 + 1)]
+            allocs += [4 * nodes, 4 * nsegs] * 2 + [4 * nodes, 4 * nodes, 4 * nsegs, 8 * 6 * nsegs * 4]
+            allocs += [8 * (math.ceil(4.0 / 3.0 * math.pi * (r + 1.0) ** 3) * np_ + 64), 4]
+            h = math.sqrt(3.0) / 2.0
+            ro, ri = r + math.sqrt(2.0) + h, max(0.0, r - h)
+            cap = (math.ceil(4.0 / 3.0 * math.pi * (ro ** 3 - ri ** 3)) * 9 + 64) * np_
+            allocs += [4, 4 * cap, cap, 8 * cap, 16 * cap, 8 * cap]
+            allocs += [4 * ncell, 4 * ((ncell + 31) // 32)] * 2
+        total = 16 << 20
+        for x in allocs:
+            g = (2 << 20) if x >= (1 << 20) else 4096
+            total += (x + g - 1) // g * g
+        return int(total)
 
 
 def draw_cases(count: int, rng: np.random.Generator, edges=(32, 48, 64, 96, 128), max_spheres=50,
@@ -147,10 +191,11 @@ class Server:
     """Continuous batching over one Ensemble."""
 
     def __init__(self, precision="fp32", budget_bytes=240e9, window=500, tol=1e-4, cap=20000,
-                 chunk=100, block=None, graph=True):
+                 chunk=100, block=None, graph=True, measure_memory=False):
         assert window % chunk == 0 and cap % chunk == 0
         self.precision, self.budget, self.window, self.tol = precision, budget_bytes, window, tol
         self.cap, self.chunk, self.block, self.graph = cap, chunk, block, graph
+        self.measure_memory = measure_memory
 
     def run(self, cases: list, clock=time.perf_counter) -> dict:
         pending = sorted(cases, key=lambda c: c.arrival)
@@ -161,6 +206,10 @@ class Server:
         if self.block:
             ens.set_block(self.block)
         used = 0
+        peak = 0
+        waits = 0
+        max_active = 0
+        mem = []                     # (estimated, measured) bytes of admitted cases
         cells_steps = 0
         t0 = clock()
         try:
@@ -169,12 +218,20 @@ class Server:
                 while pending and pending[0].arrival <= now:
                     fp = pending[0].footprint_bytes(self.precision)
                     if used + fp > self.budget and active:
+                        waits += 1
                         break        # wait for memory to free up
                     c = pending.pop(0)
+                    # measured while nothing has been freed yet (no reuse of freed memory)
+                    measure = self.measure_memory and not done
+                    free0 = device_free_bytes() if measure else 0
                     lat = make_lattice(c, self.precision)
+                    if measure:
+                        mem.append((fp, free0 - device_free_bytes()))
                     ens.add(lat, 1)
                     active[lat] = CaseRecord(case=c, admitted=clock() - t0)
                     used += fp
+                    peak = max(peak, used)
+                    max_active = max(max_active, len(active))
                 if not active:
                     time.sleep(max(0.0, min(0.01, pending[0].arrival - now)))
                     continue
@@ -203,7 +260,12 @@ class Server:
                 lat.close()
             ens.close()
         wall = clock() - t0
-        return summarize(done, wall, cells_steps)
+        out = summarize(done, wall, cells_steps)
+        out.update({"admission_waits": waits, "peak_footprint_bytes": peak, "max_active": max_active,
+                    "budget_bytes": self.budget})
+        if self.measure_memory:
+            out["footprint_estimated_measured"] = mem
+        return out
 
 
 def run_one_at_a_time(cases: list, precision="fp64", window=500, tol=1e-4, cap=20000, chunk=100,
@@ -261,35 +323,44 @@ def summarize(recs: list, wall: float, cells_steps: int) -> dict:
 def run_ensemble(cases: list, steps: int, precision="fp32", tune_steps=5, sample_every=10,
                  avg_window=500, graph=True, clock=time.perf_counter) -> dict:
     """All cases in one ensemble for `steps` steps (the autotuner's steps
-    included); per-case mean drag averaged over the last `avg_window` steps,
-    sampled every `sample_every` steps."""
+    included).  The steps between the autotuner and the last `avg_window`
+    are timed as one block (one synchronisation at its end); over the last
+    `avg_window` steps the per-case mean drag is averaged, sampled every
+    `sample_every` steps (untimed)."""
     lats = [make_lattice(c, precision) for c in cases]
     ens = Ensemble()
     ens.set_graph(graph)
     try:
         for lat in lats:
             ens.add(lat, 1)
-        block, tune = ens.autotune(tune_steps)
-        done = 3 * tune_steps
+        shape, tune = ens.autotune(tune_steps)
+        done = 12 * tune_steps
         ens.sync()
-        first = done
+        timed = max(0, steps - done - avg_window)
+        ens.host_time()
+        t0 = clock()
+        ens.step(timed)
+        host_ms, host_steps = ens.host_time()
+        ens.sync()
+        wall = clock() - t0
+        done += timed
         acc = np.zeros(len(cases))
         nacc = 0
-        wall = 0.0      # stepping time only: the drag sampling is not timed
         while done < steps:
             k = min(sample_every, steps - done)
-            t0 = clock()
             ens.step(k)
-            ens.sync()
-            wall += clock() - t0
             done += k
-            if steps - done < avg_window and k == sample_every:
+            if k == sample_every:
                 acc += np.array([mean_drag(l, c.npart) for l, c in zip(lats, cases)])
                 nacc += 1
+        ens.sync()
         cells = sum(c.cells for c in cases)
-        return {"block": block, "tune_ms_per_step": tune, "steps": steps, "graph": bool(graph),
-                "ms_per_step": 1e3 * wall / max(done - first, 1),
-                "mlups": cells * (done - first) / wall / 1e6 if wall > 0 else 0.0,
+        return {"block": shape[0], "tile_order": shape[1],
+                "tune_ms_per_step": {f"{b}/{o}": v for (b, o), v in tune.items()},
+                "steps": steps, "timed_steps": timed, "graph": bool(graph),
+                "ms_per_step": 1e3 * wall / max(timed, 1),
+                "host_ms_per_step": host_ms / max(host_steps, 1),
+                "mlups": cells * timed / wall / 1e6 if wall > 0 else 0.0,
                 "mean_drag": acc / max(nacc, 1)}
     finally:
         for lat in lats:

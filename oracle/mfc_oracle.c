@@ -12,8 +12,27 @@
  * The two agree whenever no boundary link sits on a node that is solid for
  * another sphere (near contact).
  *
- * Parity status of each function is recorded in the test-suite; the header
- * of tests/test_oracle_pins.py lists which check pins which function.
+ * What pins each function (tests/test_oracle_ref.py, tests/test_oracle_pins.py,
+ * tests/test_oracle_pins_particles.py; their headers give the detail):
+ *   orc_collision_mrt, orc_macrovar, orc_rhoupdat  the compiled reference
+ *                      (collision.f90, para.f90), mass/momentum balance, BGK limit
+ *   orc_streaming      numpy.roll, brute-force wall rule
+ *   orc_initpop/feqpnt equilibrium moments
+ *   orc_forcing_modes  solenoidal/Hermitian structure; OU increment variance,
+ *                      kurtosis, autocorrelation and stationary variance
+ *   orc_force_field    numpy.fft.irfftn of the same spectrum
+ *   orc_ranf           Park-Miller known value; shuffle-slot replay
+ *   orc_beads_links    brute-force solid and link sets, wall point on the sphere
+ *   orc_beads_collision  Lagrange interpolation on quadratic fields (all five
+ *                      branches), momentum exchange and torque sums, rotating-
+ *                      sphere torque, moving-sphere force closed form
+ *   orc_beads_lubforce hand-computed magnitudes, symmetry, wall law
+ *   orc_beads_move     ramped-force closed form, uniform acceleration
+ *   orc_beads_filling  non-equilibrium copy from ipmx on a linear field, tie
+ *                      rule, solid ipmx, the rho0 fallback, uniform equilibrium
+ *   orc_initpartvel    trilinear exactness; orc_initpartpos no overlap, seeded
+ *   orc_timestep       shear-wave decay rate, Poiseuille profile (the
+ *                      composition itself has no compiled reference here)
  *
  * Built with -ffp-contract=off: no fused multiply-add, so geometric tests
  * (node maps) are evaluated exactly as written.

@@ -27,3 +27,6 @@ echo "fortran drop-in: built $out/dropin_driver, $out/layout_check"
 "$fc" $flags -c "$here/ckpt_io.f90" -o "$out/ckpt_io.o"
 "$fc" -o "$out/ckpt_io" "$out/ckpt_io.o"
 echo "fortran drop-in: built $out/ckpt_io"
+"$fc" $flags -c "$here/big_record.f90" -o "$out/big_record.o"
+"$fc" -o "$out/big_record" "$out/big_record.o"
+echo "fortran drop-in: built $out/big_record"

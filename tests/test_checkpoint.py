@@ -1,7 +1,10 @@
 """Restart files in the reference's format (multiphase_flow_codes_amd/checkpoint.py):
 record framing byte for byte (4-byte markers, subrecords over 2 GB), files
 written by a Fortran compiler's runtime read back exactly, files written here
-read by that runtime (tests/fortran/ckpt_io.f90), and save/load round trips."""
+read by that runtime (tests/fortran/ckpt_io.f90), save/load round trips, and
+records over 2^31 bytes written by the Fortran runtimes at hand: amdflang's
+single record with an unsigned 32-bit length here; gfortran's subrecords
+where gfortran is installed."""
 import io
 import os
 import struct
@@ -29,7 +32,13 @@ def test_record_framing_bytes():
             + struct.pack("<i", 2) + bytes([8, 9]) + struct.pack("<i", -2))
     assert fh.getvalue() == want
     fh.seek(0)
-    assert ck.read_record(fh) == bytes(range(10))
+    assert ck.read_record(fh).tobytes() == bytes(range(10))
+    fh.seek(0)
+    assert ck.read_record(fh, expect=10).tobytes() == bytes(range(10))
+    fh.seek(0)
+    with pytest.raises(ValueError):
+        ck.read_record(fh, expect=9)
+    fh.seek(0, io.SEEK_END)
     with pytest.raises(EOFError):
         ck.read_record(fh)
 
@@ -134,3 +143,44 @@ def test_fortran_runtime_reads_these_files(tmp_path):
     fo = ck.load_force(tmp_path / ck.force_filename(43))
     assert fo["iseedf"] == -7 and fo["iyf"] == 5 and np.array_equal(fo["ivf"], ivf)
     assert all(np.array_equal(x, y) for x, y in zip(fo["b"], b))
+
+
+BIG_RECORD = os.path.join(HERE, "fortran", "_build", "big_record")
+
+
+def _big_record_case(tmp_path, writer):
+    import shutil
+    if shutil.disk_usage(str(tmp_path)).free < 6 << 30:
+        pytest.skip("less than 6 GB of free disk")
+    n = (1 << 28) + 17                      # 2^31 + 136 bytes of REAL(8)
+    path = str(tmp_path / "big.bin")
+    r = subprocess.run(writer + [str(n), path], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    with open(path, "rb") as fh:
+        buf = ck.read_record(fh, expect=8 * n)
+        assert fh.read(1) == b""
+    a = buf.view("<f8")
+    assert a.size == n
+    idx = np.array([0, 1, 12345, (1 << 28) - 1, 1 << 28, n - 1])
+    assert np.array_equal(a[idx], (idx + 1).astype(np.float64))
+    assert a[-1] == float(n) and np.all(np.diff(a[::1 << 20]) == float(1 << 20))
+    os.remove(path)
+
+
+def test_record_over_2gb_of_amdflang(tmp_path):
+    if not os.path.exists(BIG_RECORD):
+        pytest.skip("tests/fortran/_build/big_record not built (needs amdflang)")
+    _big_record_case(tmp_path, [BIG_RECORD])
+
+
+def test_record_over_2gb_of_gfortran(tmp_path):
+    """The GNU runtime's subrecords (the convention write_record follows)."""
+    import shutil
+    gf = shutil.which("gfortran")
+    if gf is None:
+        pytest.skip("gfortran is not installed")
+    exe = str(tmp_path / "big_record_gnu")
+    r = subprocess.run([gf, "-O1", "-o", exe, os.path.join(HERE, "fortran", "big_record.f90")],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    _big_record_case(tmp_path, [exe])

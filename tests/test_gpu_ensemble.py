@@ -2,7 +2,7 @@
 radii, densities and body forces advanced together with one grid per step
 stage give bitwise the same populations, particle state and node maps as the
 same boxes advanced one at a time with mfc_step, including a box that joins
-late and one that leaves early; the collide block autotuner keeps results
+late and one that leaves early; the collide block and tile-order autotuner keeps results
 unchanged, and so does replaying the steps as recorded HIP graphs."""
 import numpy as np
 import pytest
@@ -84,12 +84,13 @@ def test_autotune_keeps_results():
     ens = Ensemble()
     for lat in members:
         ens.add(lat, 1)
-    bs, times = ens.autotune(2)
-    assert bs in (64, 128, 256) and all(t > 0 for t in times.values())
-    ens.step(2)
+    (bs, order), times = ens.autotune(2)                # 12 x 2 steps
+    assert bs in (64, 128, 256) and order in (0, 1)
+    assert len(times) == 6 and all(t > 0 for t in times.values())
+    ens.step(3)
     ens.sync()
     for lat in alone:
-        lat.step(1, 8)
+        lat.step(1, 27)
         lat.sync()
     for a, b in zip(alone, members):
         _same(a, b)

@@ -20,7 +20,7 @@ that hold at any size:
   In the 1024^3 channel (2^30 nodes, FP32, two buffers of 81.6 GB each) the
   same comparison runs after the release and after three steps.
 * The 96-box mix (edges 32-128, 1-50 spheres each, 52.8 M nodes) in one
-  FP32 ensemble, stepped as recorded HIP graphs after the block autotuner.
+  FP32 ensemble, stepped as recorded HIP graphs after the block and tile-order autotuner.
   The largest box, the box with the most spheres and the smallest box must
   equal bitwise the same boxes stepped alone.
 """
@@ -146,8 +146,8 @@ def test_96_box_ensemble_members_equal_boxes_alone():
     ens.set_graph(True)
     for lat in lats:
         ens.add(lat, 1)
-    ens.autotune(3)                   # 3 x 3 steps
-    ens.step(21)
+    ens.autotune(2)                   # 12 x 2 steps
+    ens.step(6)
     ens.sync()
     nsteps = 30
     picks = sorted({int(np.argmax([c.cells for c in cases])),

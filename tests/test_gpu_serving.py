@@ -8,16 +8,17 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def test_serving_results_equal_alone():
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_serving_results_equal_alone(prec):
     from multiphase_flow_codes_amd import serving
     rng = np.random.default_rng(7)
     cases = serving.draw_cases(5, rng, edges=(32, 48), max_spheres=6, rad=(3.0, 5.0))
     serving.poisson_arrivals(cases, rate_per_s=20.0, rng=rng)
-    srv = serving.Server(precision="fp32", window=200, tol=1e-12, cap=600, chunk=100)
+    srv = serving.Server(precision=prec, window=200, tol=1e-12, cap=600, chunk=100)
     mix = srv.run(cases)
     assert mix["cases"] == len(cases)
     assert all(s == 600 for s in mix["steps"])
-    alone = serving.run_one_at_a_time(cases, precision="fp32", window=200, tol=1e-12, cap=600,
+    alone = serving.run_one_at_a_time(cases, precision=prec, window=200, tol=1e-12, cap=600,
                                       chunk=100)
     for c in cases:
         assert mix["drag"][c.seed] == alone["drag"][c.seed], c
@@ -34,3 +35,28 @@ def test_run_ensemble_mean_drag_finite():
     assert out["block"] in (64, 128, 256)
     assert np.all(np.isfinite(out["mean_drag"])) and np.all(out["mean_drag"] != 0.0)
     assert out["mlups"] > 0.0
+
+
+def test_admission_waits_for_the_budget_and_footprint_bounds_allocation():
+    """With a budget of about two cases, cases that arrive together wait for
+    memory: at most the budget is admitted at once (one case alone may exceed
+    it), and every case's footprint estimate bounds what its lattice
+    allocated on the device (hipMemGetInfo before and after, for the cases
+    admitted before the first one retires), within a factor of 2."""
+    from multiphase_flow_codes_amd import serving
+    rng = np.random.default_rng(9)
+    cases = serving.draw_cases(6, rng, edges=(48, 64, 80), max_spheres=12, rad=(3.0, 5.0))
+    for c in cases:
+        c.arrival = 0.0
+    budget = 2.2 * max(c.footprint_bytes("fp64") for c in cases)
+    serving.make_lattice(serving.BoxCase(n=16, npart=1, rad=3.0, rho_ratio=2.0, body_force=1e-5, seed=1),
+                         "fp64").close()          # runtime set-up outside the measurements
+    srv = serving.Server(precision="fp64", budget_bytes=budget, window=200, tol=1e-12, cap=200, chunk=100,
+                         measure_memory=True)
+    out = srv.run(cases)
+    assert out["cases"] == len(cases)
+    assert out["admission_waits"] > 0
+    assert out["max_active"] < len(cases)
+    assert out["peak_footprint_bytes"] <= budget
+    for est, got in out["footprint_estimated_measured"]:
+        assert got > 0 and got <= est <= 2 * got, (est, got)
