@@ -234,29 +234,36 @@ __global__ void __launch_bounds__(256)
   collide_node<T, R, ARR, WALLS, LAY>(a, x, (int)y, (int)z);
 }
 
-// The collide of the tiles of a list (overlapped step): each wavefront takes
-// one tile (kTileX nodes of a row) at a time.  The grid is a fixed number of
-// workgroups; the list, whose length is a device word, is split into eight
-// contiguous ranges, one per XCD (workgroup b runs on XCD b % 8), and the
-// wavefronts of an XCD sweep its range in order, so neighbouring rows meet
-// in one L2.  Every wavefront leaves the loop when the range is done.
+// The collide of the tiles of a list (overlapped step): one tile (kTileX
+// nodes of a row) per wavefront, four per workgroup, the grid sized to the
+// list.  Workgroups are dealt to the XCDs round-robin (workgroup b runs on
+// XCD b % 8), so the launch index is remapped to give each XCD a contiguous
+// range of the list, in order: neighbouring rows then meet in one L2.
+// Occupancy as the plain collide reaches it: FP32 storage with FP64
+// arithmetic held to 4 waves per SIMD with two buffers; FP64 arithmetic in
+// place held to 3 (the in-place layouts with walls otherwise take 178 VGPRs,
+// 2 waves).
+template <typename T, typename R>
+constexpr int tile_waves(int lay) {
+  return sizeof(R) == 8 ? (lay == LAY_AB ? (sizeof(T) == 4 ? 4 : 1) : 3) : 1;
+}
+
 template <typename T, typename R, bool WALLS, int LAY>
 __global__ void __launch_bounds__(256)
-    __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && sizeof(R) == 8 && LAY == LAY_AB ? 4 : 1)))
-    k_collide_tiles(CollideArgs a, const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
-                    uint32_t tpr) {
-  const uint32_t n = *count;
-  const uint32_t xcd = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
-  const uint32_t waves = (gridDim.x / kXcds) * (blockDim.x >> 6);   // wavefronts per XCD
-  const uint32_t lo = (uint32_t)((uint64_t)n * xcd / kXcds), hi = (uint32_t)((uint64_t)n * (xcd + 1) / kXcds);
-  const uint32_t lane = threadIdx.x & 63u;
-  for (uint32_t L = lo + j * (blockDim.x >> 6) + (threadIdx.x >> 6); L < hi; L += waves) {
-    const uint32_t t = list[L];
-    const uint32_t row = t / tpr;
-    const int x = (int)((t - row * tpr) * (uint32_t)kTileX + lane);
-    if (x < a.g.nx)
-      collide_node<T, R, false, WALLS, LAY>(a, x, (int)(row % (uint32_t)a.g.ny), (int)(row / (uint32_t)a.g.ny));
-  }
+    __attribute__((amdgpu_waves_per_eu(tile_waves<T, R>(LAY))))
+    k_collide_tiles(CollideArgs a, const uint32_t* __restrict__ list, uint32_t n, uint32_t tpr) {
+  uint32_t b = blockIdx.x;
+  const uint32_t per = gridDim.x / kXcds;
+  if (b < per * kXcds) b = (b % kXcds) * per + b / kXcds;
+  // the wavefront's index is uniform: keep it (and the tile, row, y, z) scalar
+  const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const uint32_t L = b * 4u + w;
+  if (L >= n) return;
+  const uint32_t t = list[L];
+  const uint32_t row = t / tpr;
+  const int x = (int)((t - row * tpr) * (uint32_t)kTileX + (threadIdx.x & 63u));
+  if (x >= a.g.nx) return;
+  collide_node<T, R, false, WALLS, LAY>(a, x, (int)(row % (uint32_t)a.g.ny), (int)(row / (uint32_t)a.g.ny));
 }
 
 // Several boxes in one grid (ensemble engine): block b belongs to box c with
@@ -544,39 +551,36 @@ static void collide_dispatch(Ctx& c, bool arrays, const CollideArgs& a) {
     collide_dispatch_lay<T, R, LAY_AB>(c, arrays, a);
 }
 
-// grid of the tile collide: 8 XCDs x 256 CUs' worth of 256-thread workgroups
-constexpr unsigned kTileGrid = 8 * 1024;
-
 template <typename T, typename R, int LAY>
-static void collide_tiles_lay(Ctx& c, const CollideArgs& a, const uint32_t* list, const uint32_t* count) {
+static void collide_tiles_lay(Ctx& c, const CollideArgs& a, const uint32_t* list, uint32_t n) {
   const uint32_t tpr = (uint32_t)((c.g.nx + kTileX - 1) / kTileX);
+  const dim3 grid((n + 3) / 4);
   if (c.g.walls_y)
-    hipLaunchKernelGGL((k_collide_tiles<T, R, true, LAY>), dim3(kTileGrid), dim3(256), 0, c.stream, a, list,
-                       count, tpr);
+    hipLaunchKernelGGL((k_collide_tiles<T, R, true, LAY>), grid, dim3(256), 0, c.stream, a, list, n, tpr);
   else
-    hipLaunchKernelGGL((k_collide_tiles<T, R, false, LAY>), dim3(kTileGrid), dim3(256), 0, c.stream, a, list,
-                       count, tpr);
+    hipLaunchKernelGGL((k_collide_tiles<T, R, false, LAY>), grid, dim3(256), 0, c.stream, a, list, n, tpr);
 }
 
 template <typename T, typename R>
-static void collide_tiles_dispatch(Ctx& c, const CollideArgs& a, const uint32_t* list, const uint32_t* count) {
+static void collide_tiles_dispatch(Ctx& c, const CollideArgs& a, const uint32_t* list, uint32_t n) {
   if (c.g.lay == LAY_AA_ODD)
-    collide_tiles_lay<T, R, LAY_AA_ODD>(c, a, list, count);
+    collide_tiles_lay<T, R, LAY_AA_ODD>(c, a, list, n);
   else if (c.g.lay == LAY_AA_EVEN)
-    collide_tiles_lay<T, R, LAY_AA_EVEN>(c, a, list, count);
+    collide_tiles_lay<T, R, LAY_AA_EVEN>(c, a, list, n);
   else
-    collide_tiles_lay<T, R, LAY_AB>(c, a, list, count);
+    collide_tiles_lay<T, R, LAY_AB>(c, a, list, n);
 }
 
-hipError_t launch_collide_tiles(Ctx& c, const uint32_t* list, const uint32_t* count, bool far) {
+hipError_t launch_collide_tiles(Ctx& c, const uint32_t* list, uint32_t n, bool far) {
+  if (n == 0) return hipSuccess;
   CollideArgs a = collide_args(c);
   if (far) a.mask = nullptr;   // no node of a far tile is solid in any map of the step
   if (c.prec == MFC_PREC_FP64)
-    collide_tiles_dispatch<double, double>(c, a, list, count);
+    collide_tiles_dispatch<double, double>(c, a, list, n);
   else if (c.prec == MFC_PREC_FP32_F64MATH)
-    collide_tiles_dispatch<float, double>(c, a, list, count);
+    collide_tiles_dispatch<float, double>(c, a, list, n);
   else
-    collide_tiles_dispatch<float, float>(c, a, list, count);
+    collide_tiles_dispatch<float, float>(c, a, list, n);
   return hipGetLastError();
 }
 

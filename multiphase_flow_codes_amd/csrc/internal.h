@@ -59,7 +59,8 @@ struct TileLists {
   int32_t* cnt = nullptr;      // near tiles per chunk
   uint32_t* off_near = nullptr;
   uint32_t* off_far = nullptr;
-  uint32_t* tot = nullptr;     // [0] near tiles, [1] far tiles
+  uint32_t* tot = nullptr;     // [0] near tiles, [1] far tiles (two list sets)
+  uint32_t* h_tot = nullptr;   // pinned host copy of tot
   uint32_t* near_list = nullptr;
   uint32_t* far_list = nullptr;
 };
@@ -186,8 +187,8 @@ struct Ctx {
   int xcd_remap = -1;               // collide tile order over the XCDs (-1: by shape; MFC_XCD_REMAP=0/1/2)
   // overlapped step (mfc_step with spheres): the particle phase of step n runs
   // on pstream while the collide of step n + 1 covers the tiles far from every
-  // sphere; MFC_OVERLAP=0 turns it off
-  int overlap = 1;
+  // sphere; opt-in with MFC_OVERLAP=1
+  int overlap = 0;
   hipStream_t pstream = nullptr;
   hipEvent_t ev_coll = nullptr, ev_tiles = nullptr, ev_part = nullptr;
   TileLists tiles;
@@ -196,9 +197,9 @@ struct Ctx {
 // ---- launchers (fluid.hip) ----
 hipError_t launch_force_table(Ctx& c, int slot, const ModeSet& m);
 hipError_t launch_collide(Ctx& c, bool arrays);
-// the collide (moments recomputed, never the arrays) of the tiles in `list`
-// (*count of them, a device word); with far set the solid mask is not read
-hipError_t launch_collide_tiles(Ctx& c, const uint32_t* list, const uint32_t* count, bool far);
+// the collide (moments recomputed, never the arrays) of the n tiles in
+// `list`; with far set the solid mask is not read
+hipError_t launch_collide_tiles(Ctx& c, const uint32_t* list, uint32_t n, bool far);
 hipError_t launch_macro(Ctx& c, bool pulled);
 hipError_t launch_rhoupdat(Ctx& c, bool pulled, double* d_maxdiff);
 hipError_t launch_initpop(Ctx& c);
