@@ -172,6 +172,19 @@ int mfc_step(mfc_ctx* ctx, int32_t istep, int32_t nsteps);
  * out[0] = sum rho, out[1..3] = sum ux, uy, uz, out[4] = sum |u|^2,
  * out[5] = max |u|^2.  Fixed reduction order: bitwise repeatable. */
 int mfc_diag_sums(mfc_ctx* ctx, double* out6);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that must not call
+ * hipMalloc): mfc_diag_sums with its first-level partials and results in a
+ * static device array of the library (6 x 1024 + 8 doubles per GPU,
+ * resolved when a context is created) instead of buffers allocated on first
+ * use.  out6 is DEVICE memory for the six values (same order as
+ * mfc_diag_sums); the work is queued on the context's stream without
+ * synchronising.  The rho/u arrays must exist already (MFC_ERR_STATE
+ * otherwise; mfc_download_macro, mfc_diag_sums or an upload create them);
+ * rho/u of a virtual macrovar are written into them as mfc_diag_sums does.
+ * Nothing is allocated or freed.  Calls that overlap on one GPU share the
+ * array and race. */
+int mfc_diag_sums_noalloc(mfc_ctx* ctx, double* out6);
 /* Kinetic-energy spectrum of statistc (BOX/saveload.f90:1273-1300): u
  * transformed with forward sign and 1/n per axis (BOX/mpifft.f90:52-108);
  * ek[ik-1] = sum over modes 0 <= kx < nx/2 (weight 1/2 at kx = 0), all ky, kz,

@@ -412,6 +412,8 @@ int mfc_create(const mfc_params* p, mfc_ctx** out) {
   CK(hipHostMalloc((void**)&c.h_status, sizeof(int32_t) * ST_COUNT, hipHostMallocDefault));
   CK(hipMalloc(&c.d_red, sizeof(double) * 8));
   CK(hipHostMalloc((void**)&c.h_red, sizeof(double) * 8, hipHostMallocDefault));
+  // this device's static scratch of the allocation-free sums
+  CK(sums_scratch(&c.sums_scratch));
   CK(hipStreamSynchronize(c.stream));
 #undef CK
   *out = ctx;
@@ -1284,6 +1286,31 @@ int mfc_diag_sums(mfc_ctx* ctx, double* out6) {
   HIPCHK(launch_field_sums(c, c.diag_part, c.diag_out));
   HIPCHK(hipMemcpyAsync(out6, c.diag_out, sizeof(double) * 6, hipMemcpyDeviceToHost, c.stream));
   HIPCHK(hipStreamSynchronize(c.stream));
+  return MFC_OK;
+}
+
+int mfc_diag_sums_noalloc(mfc_ctx* ctx, double* out6) {
+  NEEDCTX;
+  if (!out6) return MFC_ERR_ARG;
+  Ctx& c = ctx->c;
+  if (!c.rho) return fail(ctx, MFC_ERR_STATE, "diag_sums_noalloc: the rho/u arrays do not exist yet "
+                                               "(a call that returns them allocates them first)");
+  if (!c.sums_scratch) return fail(ctx, MFC_ERR_STATE, "diag_sums_noalloc: device scratch not resolved");
+  const size_t need = 6 * (size_t)diag_blocks() + 6;
+  if (need > sums_scratch_doubles())
+    return fail(ctx, MFC_ERR_CAPACITY, "diag_sums_noalloc: the partial sums exceed the static scratch");
+  if (c.mom != MOM_ARRAYS) {
+    if (c.mom != MOM_VIRTUAL || c.mom_version != c.f_version)
+      return fail(ctx, MFC_ERR_STATE, "rho/u requested after the field changed since macrovar");
+    if (c.arrays_version != c.mom_version) {
+      HIPCHK(launch_macro(c, c.phase == PHASE_POST_STREAM));
+      c.arrays_version = c.mom_version;
+    }
+  }
+  double* part = c.sums_scratch;
+  double* res = part + 6 * (size_t)diag_blocks();
+  HIPCHK(launch_field_sums(c, part, res));
+  HIPCHK(hipMemcpyAsync(out6, res, sizeof(double) * 6, hipMemcpyDeviceToDevice, c.stream));
   return MFC_OK;
 }
 

@@ -7,12 +7,25 @@
 // component at a time through a complex FP64 scratch field: radix-2 in LDS
 // for power-of-two lengths, a direct DFT otherwise; each axis is normalised
 // by 1/n as mpifft3DRC does (BOX/mpifft.f90:52-108), forward sign e^{-i}.
+#include <mutex>
+
 #include "internal.h"
 
 namespace mfc {
 
 constexpr int DB = 256;          // threads per reduction block
 constexpr int kSumBlocks = 1024; // partial sums of the first level
+
+// Scratch of the allocation-free field sums (mfc_diag_sums_noalloc): the
+// kSumBlocks x 6 first-level partials, then the 6 results it leaves before
+// they are copied out.  Every GPU the library runs on holds its own copy
+// until the process ends.  The host resolves its address once per device
+// (sums_scratch) and passes that pointer to the kernels; the array's name in
+// host code is only a stand-in.  Each call overwrites every word it reads,
+// so nothing is cleared between calls; calls that overlap on one GPU share
+// the array and race.
+constexpr size_t kSumsScratchDoubles = 6 * (size_t)kSumBlocks + 8;
+static __device__ __align__(256) double scratch[kSumsScratchDoubles];
 
 // Per-block partial sums of rho, ux, uy, uz and u.u, and the block's max |u|^2.
 __global__ void __launch_bounds__(DB)
@@ -187,6 +200,29 @@ static int ilog2_exact(int n) {
 }
 
 int diag_blocks() { return kSumBlocks; }
+
+size_t sums_scratch_doubles() { return kSumsScratchDoubles; }
+
+// Device address of `scratch` on the current device, resolved with
+// hipGetSymbolAddress the first time a context is created on that device
+// and kept per device ordinal.
+hipError_t sums_scratch(double** out) {
+  static std::mutex mu;
+  static double* ptr[kMaxDevices] = {};
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    ptr[dev] = (double*)p;
+  }
+  *out = ptr[dev];
+  return hipSuccess;
+}
 
 hipError_t launch_spectrum(Ctx& c, int nek, double2* Z, const double2* const tw[3], double* part,
                            double* acc) {

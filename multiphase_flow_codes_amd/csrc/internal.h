@@ -34,6 +34,7 @@ enum StatusWord {
 };
 
 constexpr int kForceSlots = 4;
+constexpr int kMaxDevices = 64;   // device ordinals the per-device tables cover
 // particle pair searches use a cell grid above this many particles; a cell
 // (between 1 and 4/3 times 2r + mingap wide) holds at most kCellCap centres
 // (non-overlapping spheres never come close to it; more is reported)
@@ -185,6 +186,7 @@ struct Ctx {
   double2* diag_tw[3] = {nullptr, nullptr, nullptr};   // exp(-2 pi i j / n) per axis
   void* ens = nullptr;              // the ensemble this box is a member of (its stream is on loan)
   int xcd_remap = -1;               // collide tile order over the XCDs (-1: by shape; MFC_XCD_REMAP=0/1/2)
+  double* sums_scratch = nullptr;   // this device's static scratch of mfc_diag_sums_noalloc
   // overlapped step (mfc_step with spheres): the particle phase of step n runs
   // on pstream while the collide of step n + 1 covers the tiles far from every
   // sphere; opt-in with MFC_OVERLAP=1
@@ -241,6 +243,10 @@ hipError_t launch_init_velocity(Ctx& c, const double2* amp, double scale, double
 // ---- launchers (diag.hip) ----
 int diag_blocks();
 hipError_t launch_field_sums(Ctx& c, double* part, double* out);
+// the static device scratch of the allocation-free sums: its size in
+// doubles and its address on the current device
+size_t sums_scratch_doubles();
+hipError_t sums_scratch(double** out);
 hipError_t launch_spectrum(Ctx& c, int nek, double2* Z, const double2* const tw[3], double* part,
                            double* acc);
 

@@ -100,6 +100,7 @@ SIGNATURES = {
     "mfc_get_links": [C.c_void_p, _I, _I, _D],
     "mfc_step": [C.c_void_p, C.c_int32, C.c_int32],
     "mfc_diag_sums": [C.c_void_p, _D],
+    "mfc_diag_sums_noalloc": [C.c_void_p, C.c_void_p],
     "mfc_diag_spectrum": [C.c_void_p, C.c_int32, _D],
     "mfc_set_timing": [C.c_void_p, C.c_int32],
     "mfc_get_timing": [C.c_void_p, _D, C.POINTER(C.c_int64)],
@@ -351,6 +352,12 @@ class Lattice:
         self._chk(lib().mfc_diag_sums(self.h, _dp(out)))
         return {"rho": out[0], "ux": out[1], "uy": out[2], "uz": out[3], "u2": out[4],
                 "u2max": out[5]}
+
+    def diag_sums_noalloc(self, dev_out6: int):
+        """mfc_diag_sums into six doubles of device memory at address
+        dev_out6, with the library's static scratch (allocates nothing,
+        does not synchronise)."""
+        self._chk(lib().mfc_diag_sums_noalloc(self.h, C.c_void_p(int(dev_out6))))
 
     def diag_spectrum(self, nek: int) -> np.ndarray:
         """Kinetic-energy spectrum E(ik), ik = 1..nek, computed on the device."""
