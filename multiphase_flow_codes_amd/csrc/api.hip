@@ -356,8 +356,11 @@ int mfc_create(const mfc_params* p, mfc_ctx** out) {
     // The overlapped step is opt-in: at 1024^3 with 20,000 spheres the
     // bounce-back beside the far-tile collide gets too small a share of
     // the CUs and the step ends later than the plain sequence.
+    // MFC_OVERLAP=1: the far collide beside the whole particle phase;
+    // MFC_OVERLAP=2: beside the phase after the bounce-back, which then runs
+    // with the GPU to itself
     const char* e = getenv("MFC_OVERLAP");
-    c.overlap = (e && e[0] == '1') ? 1 : 0;
+    c.overlap = (e && (e[0] == '1' || e[0] == '2')) ? e[0] - '0' : 0;
   }
   {
     // Population planes 512 B further apart than the node count: with a
@@ -452,6 +455,7 @@ int mfc_destroy(mfc_ctx* ctx) {
     if (c.ev_coll) hipEventDestroy(c.ev_coll);
     if (c.ev_tiles) hipEventDestroy(c.ev_tiles);
     if (c.ev_part) hipEventDestroy(c.ev_part);
+    if (c.ev_bb) hipEventDestroy(c.ev_bb);
     if (c.pstream) hipStreamDestroy(c.pstream);
   }
   if (c.h_status) hipHostFree(c.h_status);
@@ -1098,6 +1102,7 @@ int ensure_overlap(mfc_ctx* ctx) {
   HIPCHK(hipEventCreateWithFlags(&c.ev_coll, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&c.ev_tiles, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&c.ev_part, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&c.ev_bb, hipEventDisableTiming));
   return MFC_OK;
 }
 
@@ -1158,6 +1163,7 @@ int step_overlapped(mfc_ctx* ctx, int32_t istep, int32_t nsteps) {
       if ((he = hipStreamWaitEvent(sB, c.ev_coll, 0)) != hipSuccess) break;
       // beads_collision with beads_lubforce and the map clearing in its grid
       if ((he = launch_bounce_back(c, true)) != hipSuccess) break;
+      if ((he = hipEventRecord(c.ev_bb, sB)) != hipSuccess) break;
       c.f_version++;
       if ((prc = mfc_streaming(ctx))) break;
       if ((he = launch_move(c, it == c.p.irelease ? 1 : 0, 1, true)) != hipSuccess) break;
@@ -1185,6 +1191,7 @@ int step_overlapped(mfc_ctx* ctx, int32_t istep, int32_t nsteps) {
     // collide of step it, which stream A still runs while the host waits
     HIPCHK(hipEventSynchronize(c.ev_tiles));
     HIPCHK(hipStreamWaitEvent(sA, c.ev_tiles, 0));
+    if (c.overlap == 2) HIPCHK(hipStreamWaitEvent(sA, c.ev_bb, 0));
     if ((rc = collide_tiles(ctx, k, true, timed))) return rc;
     HIPCHK(hipStreamWaitEvent(sA, c.ev_part, 0));
     if ((rc = collide_tiles(ctx, k, false, timed))) return rc;

@@ -189,10 +189,11 @@ struct Ctx {
   double* sums_scratch = nullptr;   // this device's static scratch of mfc_diag_sums_noalloc
   // overlapped step (mfc_step with spheres): the particle phase of step n runs
   // on pstream while the collide of step n + 1 covers the tiles far from every
-  // sphere; opt-in with MFC_OVERLAP=1
+  // sphere; opt-in with MFC_OVERLAP=1 (beside the whole phase) or 2 (beside
+  // the phase after the bounce-back)
   int overlap = 0;
   hipStream_t pstream = nullptr;
-  hipEvent_t ev_coll = nullptr, ev_tiles = nullptr, ev_part = nullptr;
+  hipEvent_t ev_coll = nullptr, ev_tiles = nullptr, ev_part = nullptr, ev_bb = nullptr;
   TileLists tiles;
 };
 

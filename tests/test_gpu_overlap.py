@@ -1,6 +1,7 @@
 """The overlapped step (particle phase of step n on a second stream beside
-the collide of step n + 1 on the tiles far from every sphere) gives bitwise
-the results of the plain step sequence (MFC_OVERLAP=0): populations,
+the collide of step n + 1 on the tiles far from every sphere, from the
+phase's start with MFC_OVERLAP=1 or after its bounce-back with 2) gives
+bitwise the results of the plain step sequence (MFC_OVERLAP=0): populations,
 particle state and node maps, for both storages, both precisions, periodic
 boxes and walls, across several mfc_step calls."""
 import os
@@ -16,7 +17,7 @@ pytestmark = pytest.mark.gpu
 def _make(prm, prec, storage, overlap, pos, u):
     from multiphase_flow_codes_amd.lbm import Lattice
     old = os.environ.get("MFC_OVERLAP")
-    os.environ["MFC_OVERLAP"] = "1" if overlap else "0"
+    os.environ["MFC_OVERLAP"] = str(int(overlap))
     try:
         lat = Lattice(prm, prec, storage=storage)
     finally:
@@ -45,14 +46,15 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("n,walls,prec,storage,npart,rad", CASES)
-def test_overlapped_step_is_bitwise_the_plain_step(oracle_lib, n, walls, prec, storage, npart, rad):
+def test_overlapped_step_is_bitwise_the_plain_step(oracle_lib, n, walls, prec, storage, npart, rad, mode):
     rng = np.random.default_rng(sum(n) + npart)
     prm = small_params(n=n, npart=npart, rad=rad, walls=walls, rhopart=1.5)
     u = smooth_velocity(prm, rng, amp=0.03)
     pos = place_spheres(prm, rng, npart, gap=2.5, margin=rad + 3.0 if walls else 0.0)
-    plain = _make(prm, prec, storage, False, pos, u)
-    over = _make(prm, prec, storage, True, pos, u)
+    plain = _make(prm, prec, storage, 0, pos, u)
+    over = _make(prm, prec, storage, mode, pos, u)
     try:
         it = 1
         for chunk in (1, 5, 7):
@@ -80,7 +82,7 @@ def test_overlapped_step_timing_counts_steps(oracle_lib):
     prm = small_params(n=(96, 64, 64), npart=2, rad=4.0, rhopart=1.5)
     u = smooth_velocity(prm, rng, amp=0.02)
     pos = place_spheres(prm, rng, 2, gap=2.5)
-    lat = _make(prm, "fp32", "ab", True, pos, u)
+    lat = _make(prm, "fp32", "ab", 1, pos, u)
     try:
         lat.set_timing(1)
         lat.step(1, 6)
