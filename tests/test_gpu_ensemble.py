@@ -2,8 +2,9 @@
 radii, densities and body forces advanced together with one grid per step
 stage give bitwise the same populations, particle state and node maps as the
 same boxes advanced one at a time with mfc_step, including a box that joins
-late and one that leaves early; the collide block and tile-order autotuner keeps results
-unchanged, and so does replaying the steps as recorded HIP graphs."""
+late and one that leaves early, and for channel boxes with walls; the collide
+block and tile-order autotuner keeps results unchanged, and so does replaying
+the steps as recorded HIP graphs."""
 import numpy as np
 import pytest
 
@@ -91,6 +92,56 @@ def test_autotune_keeps_results():
     ens.sync()
     for lat in alone:
         lat.step(1, 27)
+        lat.sync()
+    for a, b in zip(alone, members):
+        _same(a, b)
+    ens.close()
+
+
+WALL_CASES = [
+    # (n, npart, rad, rhopart, body force x, seed): channel boxes, walls at y = 0 and y = ny
+    ((32, 28, 24), 2, 3.0, 2.5, 3e-5, 11),
+    ((40, 32, 36), 3, 3.5, 5.0, 1e-5, 12),
+    ((24, 24, 20), 0, 3.0, 2.5, 5e-5, 13),
+]
+
+
+def _channel_box(case, precision):
+    from multiphase_flow_codes_amd.lbm import Lattice
+    n, npart, rad, rhop, bf, seed = case
+    prm = small_params(n=n, npart=npart, rad=rad, rhopart=rhop, forcing=False, walls=True, visc=0.01,
+                       strong_forcing=False, body_force=(bf, 0.0, 0.0))
+    rng = np.random.default_rng(seed)
+    lat = Lattice(prm, precision)
+    ux, uy, uz = smooth_velocity(prm, rng, amp=0.02)
+    lat.upload_macro(None, ux, uy, uz)
+    lat.initpop()
+    lat.rhoupdat()
+    lat.macrovar()
+    if npart:
+        lat.prt_init(npart)
+        lat.prt_set("ypglb", place_spheres(prm, rng, npart, gap=2.5, margin=rad + 3.0))
+        lat.initpartvel()
+        lat.initpartomg()
+        lat.beads_links()
+    return lat
+
+
+@pytest.mark.parametrize("precision,graph", [("fp64", False), ("fp32", True)])
+def test_channel_ensemble_equals_boxes_alone(precision, graph):
+    """Boxes with walls (the channel variant) in one ensemble: the batched
+    kernels' wall variants give each box bitwise its single-box result."""
+    from multiphase_flow_codes_amd.lbm import Ensemble
+    alone = [_channel_box(c, precision) for c in WALL_CASES]
+    members = [_channel_box(c, precision) for c in WALL_CASES]
+    ens = Ensemble()
+    ens.set_graph(graph)
+    for lat in members:
+        ens.add(lat, 1)
+    ens.step(11)
+    ens.sync()
+    for lat in alone:
+        lat.step(1, 11)
         lat.sync()
     for a, b in zip(alone, members):
         _same(a, b)
