@@ -427,9 +427,9 @@ int mfc_destroy(mfc_ctx* ctx) {
   NEEDCTX;
   Ctx& c = ctx->c;
   if (c.ens) return fail(ctx, MFC_ERR_STATE, "box is an ensemble member: remove it first");
-  if (c.stream) hipStreamSynchronize(c.stream);
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
   auto F = [](void* p) {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   };
   F(c.pop[0]); F(c.pop[1]);
   F(c.rho); F(c.ux); F(c.uy); F(c.uz);
@@ -451,18 +451,18 @@ int mfc_destroy(mfc_ctx* ctx) {
   {
     TileLists& T = c.tiles;
     F(T.mark); F(T.cnt); F(T.off_near); F(T.off_far); F(T.tot); F(T.near_list); F(T.far_list);
-    if (T.h_tot) hipHostFree(T.h_tot);
-    if (c.ev_coll) hipEventDestroy(c.ev_coll);
-    if (c.ev_tiles) hipEventDestroy(c.ev_tiles);
-    if (c.ev_part) hipEventDestroy(c.ev_part);
-    if (c.ev_bb) hipEventDestroy(c.ev_bb);
-    if (c.pstream) hipStreamDestroy(c.pstream);
+    if (T.h_tot) (void)hipHostFree(T.h_tot);
+    if (c.ev_coll) (void)hipEventDestroy(c.ev_coll);
+    if (c.ev_tiles) (void)hipEventDestroy(c.ev_tiles);
+    if (c.ev_part) (void)hipEventDestroy(c.ev_part);
+    if (c.ev_bb) (void)hipEventDestroy(c.ev_bb);
+    if (c.pstream) (void)hipStreamDestroy(c.pstream);
   }
-  if (c.h_status) hipHostFree(c.h_status);
-  if (c.h_red) hipHostFree(c.h_red);
-  for (auto e : c.tpool) hipEventDestroy(e);
+  if (c.h_status) (void)hipHostFree(c.h_status);
+  if (c.h_red) (void)hipHostFree(c.h_red);
+  for (auto e : c.tpool) (void)hipEventDestroy(e);
   c.tpool.clear();
-  if (c.stream && c.own_stream) hipStreamDestroy(c.stream);
+  if (c.stream && c.own_stream) (void)hipStreamDestroy(c.stream);
   delete ctx;
   return MFC_OK;
 }
@@ -643,9 +643,9 @@ int mfc_init_velocity(mfc_ctx* ctx, const double* amp, double scale) {
   if (e == hipSuccess) e = hipMemcpyAsync(d_amp, amp, sizeof(double2) * na, hipMemcpyHostToDevice, c.stream);
   if (e == hipSuccess) e = launch_init_velocity(c, d_amp, scale, B1, B2);
   if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
-  if (d_amp) hipFree(d_amp);
-  if (B1) hipFree(B1);
-  if (B2) hipFree(B2);
+  if (d_amp) (void)hipFree(d_amp);
+  if (B1) (void)hipFree(B1);
+  if (B2) (void)hipFree(B2);
   if (e != hipSuccess) return fail(ctx, MFC_ERR_HIP, "mfc_init_velocity", e);
   c.mom = MOM_ARRAYS;
   return MFC_OK;

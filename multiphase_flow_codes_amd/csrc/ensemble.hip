@@ -317,13 +317,13 @@ int ens_chunk(mfc_ens* e, int n) {
       const int rc = enqueue();
       const hipError_t end = hipStreamEndCapture(e->stream, &g);
       if (rc) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);
         return rc;
       }
       if (end != hipSuccess) return efail(e, MFC_ERR_HIP, "hipStreamEndCapture", end);
       hipGraphExec_t ex = nullptr;
       const hipError_t inst = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-      hipGraphDestroy(g);
+      (void)hipGraphDestroy(g);
       if (inst != hipSuccess) return efail(e, MFC_ERR_HIP, "hipGraphInstantiate", inst);
       e->graphs.push_back({sig, ex});
       rec = &e->graphs.back();
@@ -363,7 +363,7 @@ int mfc_ens_create(mfc_ens** out) {
 
 int mfc_ens_destroy(mfc_ens* e) {
   if (!e) return MFC_ERR_ARG;
-  if (e->stream) hipStreamSynchronize(e->stream);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
   for (Member& mb : e->m) {
     mb.box->c.stream = mb.own_stream;
     mb.box->c.ens = nullptr;
@@ -371,14 +371,14 @@ int mfc_ens_destroy(mfc_ens* e) {
   e->m.clear();
   // graphs first: their nodes point into the table buffers freed below
   for (auto& g : e->graphs)
-    if (g.exec) hipGraphExecDestroy(g.exec);
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
   e->graphs.clear();
   for (int r = 0; r < kRing; ++r) {
-    if (e->h_tab[r]) hipHostFree(e->h_tab[r]);
-    if (e->h_ev[r]) hipEventDestroy(e->h_ev[r]);
+    if (e->h_tab[r]) (void)hipHostFree(e->h_tab[r]);
+    if (e->h_ev[r]) (void)hipEventDestroy(e->h_ev[r]);
   }
-  if (e->d_tab) hipFree(e->d_tab);
-  if (e->stream) hipStreamDestroy(e->stream);
+  if (e->d_tab) (void)hipFree(e->d_tab);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return MFC_OK;
 }
@@ -504,8 +504,8 @@ int mfc_ens_autotune(mfc_ens* e, int32_t steps, int32_t* chosen, double* ms_per_
       best_order = k / 3;
     }
   }
-  hipEventDestroy(t0);
-  hipEventDestroy(t1);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
   e->collide_bs = best_bs;
   e->collide_remap = best_order;
   if (chosen) {
