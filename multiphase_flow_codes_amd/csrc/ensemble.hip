@@ -479,9 +479,17 @@ int mfc_ens_autotune(mfc_ens* e, int32_t steps, int32_t* chosen, double* ms_per_
   if (!e || steps < 1) return MFC_ERR_ARG;
   if (e->m.empty()) return efail(e, MFC_ERR_STATE, "autotune of an empty ensemble");
   const int bsz[3] = {64, 128, 256};
-  hipEvent_t t0, t1;
-  EHIP(hipEventCreate(&t0));
-  EHIP(hipEventCreate(&t1));
+  // the timing events, destroyed on every return
+  struct Events {
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    ~Events() {
+      if (t0) (void)hipEventDestroy(t0);
+      if (t1) (void)hipEventDestroy(t1);
+    }
+  } ev;
+  EHIP(hipEventCreate(&ev.t0));
+  EHIP(hipEventCreate(&ev.t1));
+  hipEvent_t t0 = ev.t0, t1 = ev.t1;
   int best_bs = 256, best_order = 0;
   float best_ms = 1e30f;
   for (int k = 0; k < 6; ++k) {
@@ -504,8 +512,6 @@ int mfc_ens_autotune(mfc_ens* e, int32_t steps, int32_t* chosen, double* ms_per_
       best_order = k / 3;
     }
   }
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
   e->collide_bs = best_bs;
   e->collide_remap = best_order;
   if (chosen) {
