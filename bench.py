@@ -13,7 +13,7 @@ buffers (`value`), the plain FP64 path with one buffer updated in place
 beside it (two FP64 buffers do not fit), the HBM roofline fraction of the
 dominant kernel (collide-stream, timed with HIP events on its own stream)
 and of the whole step, FP32-vs-FP64 accuracy after 500 steps with an
-FP64-vs-perturbed-FP64 control, the 256^3 box (config 2) with its own
+FP64-vs-perturbed-FP64 control, the 256^3 box with 300 spheres with its own
 accuracy legs, the 96-box ensemble, the serving loop, and the FP64 C oracle
 timed on a bounded sample (a 256^3 section of the channel) on the host cores.
 """
@@ -549,7 +549,7 @@ def main():
 
 
 def run_box256(args):
-    """Config 2: the 256^3 periodic box with 300 spheres, FP32, FP64 and
+    """box256: the 256^3 periodic box with 300 spheres, FP32, FP64 and
     FP32-storage/FP64-collision throughput, and their accuracy against FP64
     after args.accuracy_steps steps with the perturbed-FP64 control."""
     from multiphase_flow_codes_amd import workloads

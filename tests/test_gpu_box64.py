@@ -1,4 +1,4 @@
-"""Config 1 on the GPU and every bounce-back branch.
+"""The 64^3 box with eight spheres on the GPU, and every bounce-back branch.
 
 * box64: the 64^3 periodic box, nu = 0.005, forced on |k| <= sqrt 2, eight
   spheres of radius 4 (density ratio 5, seeded non-overlapping positions
@@ -27,7 +27,7 @@ def rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
-def test_box64_config_matches_oracle(oracle_lib):
+def test_box64_100_steps_match_oracle(oracle_lib):
     from multiphase_flow_codes_amd import workloads
     from oracle.pyoracle import Oracle
     case = workloads.BOX64
