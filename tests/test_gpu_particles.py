@@ -154,6 +154,60 @@ def test_moving_spheres_refill_nodes(oracle_lib):
     assert np.all(np.isfinite(orc.ypglb))
 
 
+@pytest.mark.parametrize("fused", [False, True])
+def test_spheres_crossing_periodic_faces_match_oracle(oracle_lib, fused):
+    """Spheres that move across the periodic faces while stepping: the wrap
+    of beads_redistribute, nearest-image geometry for links, bounce-back and
+    refill on both sides of each face, against the oracle every step."""
+    rng = np.random.default_rng(21)
+    prm = small_params(n=(28, 24, 26), npart=3, rad=3.5, rhopart=5.0, strong_forcing=False)
+    lat, orc = _pair(prm)
+    start_both(prm, lat, orc, rng, amp=0.0)
+    orc.macrovar()
+    lat.macrovar()
+    # centres 0.05-0.2 inside a face (or a corner), moving out through it
+    pos = np.array([[prm.nx - 0.05, 11.3, 12.6],
+                    [13.7, 0.12, prm.nz - 0.2],
+                    [0.08, prm.ny - 0.1, 0.15]])
+    vel = np.array([[0.06, 0.01, -0.02],
+                    [0.015, -0.05, 0.045],
+                    [-0.04, 0.035, -0.05]])
+    orc.ypglb[:] = pos
+    orc.yp[:] = pos
+    orc.wp[:] = vel
+    orc.beads_links()
+    lat.prt_set("ypglb", pos)
+    lat.prt_set("wp", vel)
+    lat.beads_links()
+    for it in range(2, 10):     # past the release step: keep the given velocities
+        orc.S.istep = it
+        orc.call("timestep")
+        if fused:
+            lat.step(it, 1)
+        else:
+            lat.forcing()
+            lat.collision_mrt()
+            lat.beads_collision()
+            lat.streaming()
+            lat.beads_lubforce()
+            lat.beads_move(it)
+            lat.beads_redistribute(it)
+            lat.beads_links()
+            lat.beads_filling()
+            lat.macrovar()
+        lat.sync()
+        assert rel(lat.download_f(), orc.f) < TOL64, it
+        assert rel(lat.prt_get("ypglb"), orc.ypglb) < TOL64, it
+        ib, isn = lat.get_maps(0)
+        assert np.array_equal(ib, orc.ibnodes) and np.array_equal(isn, orc.isnodes), it
+    # every centre went through its face and was wrapped back into the box
+    yg = lat.prt_get("ypglb")
+    assert yg[0, 0] < 1.0 and yg[1, 1] > prm.ny - 1.0 and yg[1, 2] < 1.0
+    assert yg[2, 0] > prm.nx - 1.0 and yg[2, 2] > prm.nz - 1.0
+    assert lat.status()["refill"] > 0
+    assert rel(lat.prt_get("fHIp"), orc.fHIp) < 1e-11
+
+
 def test_multistep_call_equals_fine_grained(oracle_lib):
     """One mfc_step call over several steps (lubrication and map clearing in
     the bounce-back grids, move and refill search fused) gives bitwise the
