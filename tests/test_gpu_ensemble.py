@@ -146,3 +146,51 @@ def test_channel_ensemble_equals_boxes_alone(precision, graph):
     for a, b in zip(alone, members):
         _same(a, b)
     ens.close()
+
+
+def test_membership_errors_are_reported():
+    """The engine refuses what it cannot batch, with a message and without
+    touching the boxes: a second precision, in-place storage, a box twice, a
+    box in two ensembles, a box driven by the forcing process, and the
+    autotuner on an empty ensemble; a refused box still steps alone."""
+    from multiphase_flow_codes_amd.lbm import Ensemble, Lattice, MfcError
+    a32 = _box(CASES[0], "fp32")
+    b64 = _box(CASES[1], "fp64")
+    z = np.zeros((24, 24, 24))
+
+    def ready(lat):
+        lat.upload_macro(None, z, z, z)
+        lat.initpop()
+        lat.rhoupdat()
+        lat.macrovar()
+        return lat
+
+    aa = ready(Lattice(small_params(n=(24, 24, 24), forcing=False, strong_forcing=False), "fp32",
+                       storage="aa"))
+    forced = ready(Lattice(small_params(n=(24, 24, 24)), "fp32"))
+    ens, other = Ensemble(), Ensemble()
+    try:
+        with pytest.raises(MfcError, match="empty ensemble"):
+            ens.autotune(1)
+        ens.add(a32, 1)
+        with pytest.raises(MfcError, match="one precision"):
+            ens.add(b64, 1)
+        with pytest.raises(MfcError, match="already a member"):
+            ens.add(a32, 1)
+        with pytest.raises(MfcError, match="another ensemble"):
+            other.add(a32, 1)
+        with pytest.raises(MfcError, match="two-buffer storage"):
+            ens.add(aa, 1)
+        with pytest.raises(MfcError, match="body force only"):
+            ens.add(forced, 1)
+        assert len(ens.members) == 1
+        ens.step(2)
+        ens.sync()
+        b64.step(1, 2)       # the refused box steps alone as before
+        b64.sync()
+        assert np.isfinite(b64.download_f()).all()
+    finally:
+        ens.close()
+        other.close()
+        for lat in (a32, b64, aa, forced):
+            lat.close()
