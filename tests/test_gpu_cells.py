@@ -95,3 +95,30 @@ def test_overlap_found_through_cells():
     lat.prt_set("ypglb", pos)
     with pytest.raises(MfcError, match="overlap at istep = 5"):
         lat.beads_redistribute(5)     # synchronises, as the reference's fatal check
+
+
+@pytest.mark.parametrize("bad", ["beyond_wall", "nan"])
+def test_sphere_outside_the_box_is_a_fault(bad):
+    """A centre beyond a channel wall or not finite: the links scan skips
+    the sphere (its box check) and the next synchronisation reports the
+    fault, as the reference stops on a sphere it cannot place."""
+    from multiphase_flow_codes_amd.lbm import Lattice, MfcError
+    from helpers import small_params
+    prm = small_params(n=(32, 24, 28), npart=2, rad=3.0, walls=True, forcing=False, strong_forcing=False)
+    lat = Lattice(prm, "fp64")
+    z = np.zeros((prm.nz, prm.ny, prm.nx))
+    lat.upload_macro(None, z, z, z)
+    lat.initpop()
+    lat.rhoupdat()
+    lat.macrovar()
+    lat.prt_init(prm.npart)
+    pos = np.array([[10.3, 12.1, 9.7], [20.5, prm.ny + 5.0, 17.2]])
+    if bad == "nan":
+        pos[1] = [20.5, 12.0, np.nan]
+    lat.prt_set("ypglb", pos)
+    try:
+        with pytest.raises(MfcError, match="outside the box"):
+            lat.beads_links()
+            lat.sync()
+    finally:
+        lat.close()
