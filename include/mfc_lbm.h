@@ -163,7 +163,11 @@ int mfc_get_links(mfc_ctx* ctx, int32_t* ilink, int32_t* mlink, double* alink);
 
 /* ---- whole step ---------------------------------------------------------- */
 /* nsteps passes of the time-loop body (BOX/main.f90:172-188) starting at step
- * number istep (the release at istep==irelease is the caller's, as in main). */
+ * number istep (the release at istep==irelease is the caller's, as in main).
+ * With MFC_OVERLAP=1 or 2 in the environment at mfc_create, the particle
+ * phase of each step runs on a second stream beside the next collide of the
+ * lattice tiles far from every sphere; results are bitwise the same, and a
+ * sphere moving a node or more in one step is then reported as a fault. */
 int mfc_step(mfc_ctx* ctx, int32_t istep, int32_t nsteps);
 /* Accumulated device time of the collide-stream kernel while timing is on. */
 /* ---- FP64 diagnostics on the device ---------------------------------------- */
