@@ -389,21 +389,19 @@ __device__ __forceinline__ void block_place2(bool fa, bool fb, int (*wt)[PB / 64
 
 // beads_links, first half of the map rebuild (partlib.f90:945-949): reset the
 // nodes the same map buffer marked two steps earlier, from that build's solid
-// lists.  A node claimed by several spheres is reset by its owner (the
-// highest id, the reference's isnodes) only.
+// lists.  Every marked node of the buffer is to be reset before any new mark
+// (the marks come in a later grid), so each listed node's id and its whole
+// 32-node mask word are stored as zero: no read, no atomic; a word or a node
+// listed by several spheres receives the same zero from each.
 __device__ void clear_segment(int b, int nseg, int seg_cap, const uint32_t* __restrict__ sol_list,
                               const int32_t* __restrict__ sol_cnt, int32_t* __restrict__ isn,
                               uint32_t* __restrict__ mask) {
-  const int p = b / nseg;
   const int cnt = sol_cnt[b];
   const uint32_t* lst = sol_list + (size_t)b * seg_cap;
-  for (int e0 = 0; e0 < cnt; e0 += PB) {
-    const int e = e0 + (int)threadIdx.x;
-    const bool act = e < cnt;
-    const uint32_t n = act ? lst[e] : 0u;
-    const bool mine = act && isn[n] == p + 1;
-    if (mine) isn[n] = 0;
-    mask_update(mask, mine, n, false);
+  for (int e = (int)threadIdx.x; e < cnt; e += PB) {
+    const uint32_t n = lst[e];
+    isn[n] = 0;
+    mask[n >> 5] = 0u;
   }
 }
 
