@@ -1128,7 +1128,10 @@ __device__ void fill_find_segment(int b, const FillArgs& fa, int nseg, int32_t* 
   for (int e0 = 0; e0 < cnt; e0 += PB) {
     const int e = e0 + (int)threadIdx.x;
     const uint32_t n = e < cnt ? lst[e] : 0u;
-    const bool fill = e < cnt && fa.isn_old[n] == p + 1 && !bit_at(fa.mask_new, n);
+    // the new map's bit first: nearly every node stays solid, and only the
+    // uncovered ones need the owner test (a node of several spheres is
+    // listed once, by its owner)
+    const bool fill = e < cnt && !bit_at(fa.mask_new, n) && fa.isn_old[n] == p + 1;
     // one atomic per wavefront for the list slots
     const uint64_t bal = __ballot(fill);
     if (bal == 0ull) continue;
