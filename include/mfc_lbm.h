@@ -152,6 +152,18 @@ int mfc_initpartvel(mfc_ctx* ctx);      /* BOX/partlib.f90:177-263 */
 int mfc_initpartomg(mfc_ctx* ctx);      /* BOX/partlib.f90:837-931 (spin set to 0, :929) */
 int mfc_beads_links(mfc_ctx* ctx);      /* BOX/partlib.f90:934-1058 */
 int mfc_beads_collision(mfc_ctx* ctx);  /* BOX/partlib.f90:1064-1717 */
+/* Order of the bounce-back links (BOX/partlib.f90:1363-1621), for
+ * mfc_beads_collision and mfc_step.  MFC_BB_SNAPSHOT (the default) evaluates
+ * every link against one post-collision field.  MFC_BB_REFERENCE reproduces
+ * the reference's one-rank loop, which stores each link's value into the
+ * solid node at once: a link whose own node is solid for another sphere
+ * (spheres closer than about a node) then reads the value an earlier link
+ * stored there.  The two differ only for such contact links (counted in
+ * status word 4); the reference order evaluates them in one workgroup and
+ * holds at most 65536 of them (MFC_ERR_CAPACITY beyond). */
+#define MFC_BB_SNAPSHOT 0
+#define MFC_BB_REFERENCE 1
+int mfc_set_bb_order(mfc_ctx* ctx, int32_t order);
 int mfc_beads_lubforce(mfc_ctx* ctx);   /* BOX/partlib.f90:2211-2292 */
 int mfc_beads_move(mfc_ctx* ctx, int32_t istep);         /* BOX/partlib.f90:2295-2394 */
 int mfc_beads_redistribute(mfc_ctx* ctx, int32_t istep); /* BOX/partlib.f90:2397-2648 */

@@ -28,6 +28,8 @@ using namespace mfc;
 namespace {
 
 constexpr int kTimingPool = 8192;
+// contact links the reference-order bounce-back can hold (48 B each)
+constexpr int64_t kOrderCap = int64_t(1) << 16;
 
 int fail(mfc_ctx* ctx, int code, const char* what, hipError_t e = hipSuccess) {
   if (ctx) {
@@ -288,6 +290,11 @@ int check_faults(mfc_ctx* ctx) {
     return fail(ctx, MFC_ERR_OVERLAP, buf);
   }
   if (s[ST_LINK_OVERFLOW] > 0) return fail(ctx, MFC_ERR_CAPACITY, "fault: boundary-link pool exceeded");
+  if (s[ST_ORDER_CAP] > 0) {
+    snprintf(buf, sizeof buf, "fault: more contact links than the reference-order bounce-back holds (%lld)",
+             (long long)kOrderCap);
+    return fail(ctx, MFC_ERR_CAPACITY, buf);
+  }
   if (s[ST_CELL_OVERFLOW] > 0)
     return fail(ctx, MFC_ERR_CAPACITY, "fault: too many particles in one pair-search cell (overlapping particles)");
   if (s[ST_NONFINITE] > 0 || s[ST_BOX_FAULT] > 0)
@@ -444,7 +451,7 @@ int mfc_destroy(mfc_ctx* ctx) {
   F(P.lk_node); F(P.lk_dir); F(P.lk_cnt); F(P.lk_off); F(P.lk_part);
   F(P.sol_list[0]); F(P.sol_list[1]); F(P.sol_cnt[0]); F(P.sol_cnt[1]);
   F(P.sh_node); F(P.sh_hits); F(P.sh_cnt);
-  F(P.fill_list); F(P.fill_count); F(P.cell_cnt); F(P.cell_list);
+  F(P.fill_list); F(P.fill_count); F(P.cell_cnt); F(P.cell_list); F(P.ord);
   F(c.d_status); F(c.d_red); F(c.staging);
   F(c.diag_part); F(c.diag_out); F(c.diag_Z);
   for (int a = 0; a < 3; ++a) F(c.diag_tw[a]);
@@ -1347,6 +1354,21 @@ int mfc_diag_spectrum(mfc_ctx* ctx, int32_t nek, double* ek) {
   HIPCHK(launch_spectrum(c, nek, c.diag_Z, tw, c.diag_part, c.diag_out));
   HIPCHK(hipMemcpyAsync(ek, c.diag_out, sizeof(double) * nek, hipMemcpyDeviceToHost, c.stream));
   HIPCHK(hipStreamSynchronize(c.stream));
+  return MFC_OK;
+}
+
+int mfc_set_bb_order(mfc_ctx* ctx, int32_t order) {
+  NEEDCTX;
+  Ctx& c = ctx->c;
+  if (order != MFC_BB_SNAPSHOT && order != MFC_BB_REFERENCE)
+    return fail(ctx, MFC_ERR_ARG, "bounce-back order must be MFC_BB_SNAPSHOT or MFC_BB_REFERENCE");
+  if (c.ens) return fail(ctx, MFC_ERR_STATE, "box is an ensemble member: remove it first");
+  Particles& P = c.pt;
+  if (order == MFC_BB_REFERENCE && !P.ord) {
+    HIPCHK(hipMalloc(&P.ord, sizeof(OrderEntry) * kOrderCap));
+    P.ord_cap = kOrderCap;
+  }
+  P.bb_order = order;
   return MFC_OK;
 }
 

@@ -30,6 +30,7 @@ enum StatusWord {
   ST_BOX_FAULT = 9,      // particle boxes that did not fit the lattice
   ST_CELL_OVERFLOW = 10, // pair-search cells or neighbour lists over capacity
   ST_FAST = 11,          // sphere steps that moved a centre a node or more
+  ST_ORDER_CAP = 12,     // contact links beyond the reference-order table (not evaluated)
   ST_COUNT = 16
 };
 
@@ -86,6 +87,20 @@ struct ForceTables {
   double* phz = nullptr;
 };
 
+// One contact link of the reference-order bounce-back.  dep[0] is the link
+// that writes f_q at this link's node x (from x + c_q) and dep[1] the one that
+// writes f_qbar there (from x - c_q, read only when alpha > 1/2), each kept
+// only if the reference's loop runs it earlier and writes x in place: its
+// link index, with kOrderContact set when that link is itself a contact link
+// (its value is then resolved alongside), or -1.  val holds the value a
+// non-contact writer stores.
+constexpr int64_t kOrderContact = int64_t(1) << 62;
+struct OrderEntry {
+  int64_t dep[2];
+  double val[2];
+  double f9, ff1;   // this link's next values during resolution
+};
+
 struct Particles {
   int np = 0, np_max = 0;
   // (3, np) column-major, c + 3*p
@@ -121,6 +136,11 @@ struct Particles {
   double2* lk_out = nullptr;   // (f9, ff1) of contact links, taken before the scatter
   uint64_t* ct_list = nullptr; // contact links of the last build: (particle << 40) | link
   int32_t* ct_count = nullptr;
+  // bounce-back order (MFC_BB_SNAPSHOT or MFC_BB_REFERENCE) and, for the
+  // reference order, the per-contact-link table of its evaluation
+  int bb_order = 0;
+  OrderEntry* ord = nullptr;
+  int64_t ord_cap = 0;
   int32_t* lk_cnt = nullptr;   // links per segment
   int64_t* lk_part = nullptr;  // per-chunk sums of the segment counts (multi-block scan)
   int64_t* lk_off = nullptr;   // link offset per segment (np * nseg + 1)

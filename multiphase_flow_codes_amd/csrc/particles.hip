@@ -754,15 +754,18 @@ __global__ void __launch_bounds__(PB)
 // rigid-body velocity, then the interpolated value f9 sent back into the
 // fluid and ff1 = f_q(x) post-collision; ff1 + f9 is the link's momentum
 // exchange.  (xx0, yy0, zz0) is the wall point relative to the centre.
-// With `gather` false only the geometry is evaluated.
-template <typename T>
+// With `gather` false only the geometry is evaluated.  With kSub the values
+// read at x itself can be replaced: f_q(x) by sub_q (has_q) and, for
+// alpha > 1/2, f_qbar(x) by sub_qbar (has_qbar).
+template <typename T, bool kSub = false>
 __device__ __forceinline__ void link_bounce(const T* A, const Grid& g, const PGeom& G,
                                             const uint32_t* __restrict__ mask, uint32_t n, int q,
                                             double alpha, double xcp, double ycp, double zcp,
                                             double w1, double w2, double w3, double o1, double o2,
                                             double o3, double ww1, double ww2, bool gather,
                                             double& ff1, double& f9, double& xx0, double& yy0,
-                                            double& zz0) {
+                                            double& zz0, bool has_q = false, double sub_q = 0.0,
+                                            bool has_qbar = false, double sub_qbar = 0.0) {
   int x, y, z;
   uint32_t row;
   node_coords(n, g, x, y, z, row);
@@ -794,8 +797,12 @@ __device__ __forceinline__ void link_bounce(const T* A, const Grid& g, const PGe
   const double wt = q <= 6 ? ww1 : ww2;
   const bool lo = alpha <= 0.5;
   ff1 = post_value(A, g, q, n);
-  const double v1 = post_value(A, g, lo ? q : qo, lo ? m1 : n);
+  double v1 = post_value(A, g, lo ? q : qo, lo ? m1 : n);
   const double v2 = post_value(A, g, lo ? q : qo, lo ? m2 : m1);
+  if (kSub) {
+    if (has_q) ff1 = sub_q;
+    if (has_qbar && !lo) v1 = sub_qbar;
+  }
   const bool sol1 = out1 || bit_at(mask, m1);
   const bool sol2 = out2 || bit_at(mask, m2);
   if (lo)
@@ -803,6 +810,126 @@ __device__ __forceinline__ void link_bounce(const T* A, const Grid& g, const PGe
                            0.0, 0.0, wt, uwpro);
   else
     f9 = bounce_back_value(alpha, ff1, 0.0, 0.0, sol1, true, v1, sol1 ? 0.0 : v2, wt, uwpro);
+}
+
+// Contact links in the reference's order (one workgroup).  The reference
+// runs its links in (k, j, i, q) loop order and stores each value at once into
+// the solid node x + c (partlib.f90:1363-1621).  A link reads f_q and, for
+// alpha > 1/2, f_qbar at its own node x; every other value it uses is at a
+// fluid node, which no link writes.  So only a contact link (x solid for
+// another sphere) can read a stored value: the one its writer stored at x if
+// the loop ran that writer earlier and the store reached x in place, not
+// through the halo copy of a periodic y or z face, which the reference
+// applies after the loop (partlib.f90:1605-1621, 2101-2171).  Non-contact
+// links keep their snapshot values; the contact links' values are the fixed
+// point of f9(C) = F(the values of C's earlier writers), found by Jacobi
+// passes over the contact list until no value changes (writers run earlier,
+// so the dependencies are acyclic and pass m settles chains of length m).
+// The writer of f_q(x) is the link of x's owner at x + c_q along -c_q, the
+// writer of f_qbar(x) its link at x - c_q along c_q.
+template <typename T>
+__device__ void bb_contact_ordered(const BounceArgs& a) {
+  __shared__ int changed;
+  const T* A = (const T*)a.A;
+  const Grid& g = a.g;
+  const int64_t total = min((int64_t)*a.ct_count, a.cap);
+  if (total > a.ord_cap) {
+    if (threadIdx.x == 0) atomicAdd(&a.status[ST_ORDER_CAP], 1);
+    return;
+  }
+  for (int64_t e = threadIdx.x; e < total; e += PB) {
+    const uint64_t ent = a.ct_list[e];
+    const int p = (int)(ent >> 40);
+    const int64_t l = (int64_t)(ent & ((1ull << 40) - 1));
+    OrderEntry o;
+    o.dep[0] = o.dep[1] = -1;
+    o.val[0] = o.val[1] = o.f9 = o.ff1 = 0.0;
+    const uint32_t n = l < a.cap ? a.lk_node[l] : 0u;
+    const int q = l < a.cap ? a.lk_dir[l] : 0;
+    if (p < a.np && l < a.cap && n < g.ncell && q >= 1 && q < NPOP) {
+      const double alpha = a.lk_alpha[l];
+      double ff1, f9, xx0, yy0, zz0;
+      link_bounce<T>(A, g, a.G, a.mask, n, q, alpha, a.ypglb[3 * p], a.ypglb[3 * p + 1],
+                     a.ypglb[3 * p + 2], a.wp[3 * p], a.wp[3 * p + 1], a.wp[3 * p + 2], a.omgp[3 * p],
+                     a.omgp[3 * p + 1], a.omgp[3 * p + 2], a.ww1, a.ww2, true, ff1, f9, xx0, yy0, zz0);
+      a.lk_out[l] = make_double2(f9, ff1);
+      int x, y, z;
+      uint32_t row;
+      node_coords(n, g, x, y, z, row);
+      const int own = a.isn[n] - 1;
+      for (int s = 0; s < 2 && own >= 0 && own < a.np; ++s) {
+        if (s == 1 && !(alpha > 0.5)) break;
+        const int sg = s == 0 ? 1 : -1;
+        const int xw = x + sg * cx(q), yw = y + sg * cy(q), zw = z + sg * cz(q);
+        if (yw < 0 || yw >= g.ny || zw < 0 || zw >= g.nz) continue;   // stored through a halo
+        const uint32_t nw = (uint32_t)wrap1(xw, g.nx) + (uint32_t)g.nx * ((uint32_t)yw + (uint32_t)g.ny * (uint32_t)zw);
+        if (nw > n) continue;   // the loop runs the writer later
+        const int qw = s == 0 ? opp(q) : q;
+        const int64_t lo = a.off[(int64_t)own * a.nseg];
+        const int64_t hi = min(a.off[(int64_t)(own + 1) * a.nseg], a.cap);
+        int64_t lw = -1;
+        for (int64_t k = lo; k < hi; ++k)
+          if (a.lk_node[k] == nw && a.lk_dir[k] == qw) {
+            lw = k;
+            break;
+          }
+        if (lw < 0) continue;
+        if (bit_at(a.mask, nw)) {
+          o.dep[s] = lw | kOrderContact;
+        } else {
+          double wf1, wf9, wx, wy, wz;
+          link_bounce<T>(A, g, a.G, a.mask, nw, qw, a.lk_alpha[lw], a.ypglb[3 * own],
+                         a.ypglb[3 * own + 1], a.ypglb[3 * own + 2], a.wp[3 * own], a.wp[3 * own + 1],
+                         a.wp[3 * own + 2], a.omgp[3 * own], a.omgp[3 * own + 1], a.omgp[3 * own + 2],
+                         a.ww1, a.ww2, true, wf1, wf9, wx, wy, wz);
+          o.dep[s] = lw;
+          o.val[s] = (double)(T)wf9;   // what the writer stores
+        }
+      }
+    }
+    a.ord[e] = o;
+  }
+  for (int64_t pass = 0; pass <= total + 1; ++pass) {
+    if (threadIdx.x == 0) changed = 0;
+    __syncthreads();   // phase-1 entries and the previous pass's values are visible
+    for (int64_t e = threadIdx.x; e < total; e += PB) {
+      OrderEntry& o = a.ord[e];
+      if (o.dep[0] < 0 && o.dep[1] < 0) continue;
+      const uint64_t ent = a.ct_list[e];
+      const int p = (int)(ent >> 40);
+      const int64_t l = (int64_t)(ent & ((1ull << 40) - 1));
+      double v[2];
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int64_t d = o.dep[s];
+        v[s] = d < 0 ? 0.0 : ((d & kOrderContact) ? (double)(T)a.lk_out[d & ~kOrderContact].x : o.val[s]);
+      }
+      double ff1, f9, xx0, yy0, zz0;
+      link_bounce<T, true>(A, g, a.G, a.mask, a.lk_node[l], a.lk_dir[l], a.lk_alpha[l], a.ypglb[3 * p],
+                           a.ypglb[3 * p + 1], a.ypglb[3 * p + 2], a.wp[3 * p], a.wp[3 * p + 1],
+                           a.wp[3 * p + 2], a.omgp[3 * p], a.omgp[3 * p + 1], a.omgp[3 * p + 2],
+                           a.ww1, a.ww2, true, ff1, f9, xx0, yy0, zz0, o.dep[0] >= 0, v[0],
+                           o.dep[1] >= 0, v[1]);
+      o.f9 = f9;
+      o.ff1 = ff1;
+    }
+    __syncthreads();   // every next value is computed before any is stored
+    for (int64_t e = threadIdx.x; e < total; e += PB) {
+      const OrderEntry& o = a.ord[e];
+      if (o.dep[0] < 0 && o.dep[1] < 0) continue;
+      const int64_t l = (int64_t)(a.ct_list[e] & ((1ull << 40) - 1));
+      const double2 cur = a.lk_out[l];
+      if (__double_as_longlong(cur.x) != __double_as_longlong(o.f9) ||
+          __double_as_longlong(cur.y) != __double_as_longlong(o.ff1)) {
+        a.lk_out[l] = make_double2(o.f9, o.ff1);
+        changed = 1;
+      }
+    }
+    __syncthreads();
+    const int more = changed;
+    __syncthreads();   // every thread has read the flag before it is reset
+    if (!more) break;
+  }
 }
 
 // Pre-pass of the bounce-back.  Contact links (node x solid for another
@@ -821,6 +948,10 @@ __device__ __forceinline__ void bb_contact_body(int b_, const BounceArgs& a) {
       lub_cells(r * PB + (int)threadIdx.x, a.ypglb, a.np, a.G, a.lub, a.status);
     else
       lub_block(r, a.ypglb, a.np, a.G, a.lub);
+    return;
+  }
+  if (a.ref_order) {
+    if (b_ == 0) bb_contact_ordered<T>(a);
     return;
   }
   const int nblk = a.nct;
@@ -1649,7 +1780,9 @@ BounceArgs bounce_args(Ctx& c, bool fused, int nct, int* bb_blocks, int* pre_blo
   a.ww1 = c.p.ww1; a.ww2 = c.p.ww2; a.seg_ft = P.seg_ft; a.status = c.d_status;
   a.nsegs = P.np * P.nseg; a.np = P.np; a.lub = lub_args(c);
   a.split = P.bb_split;
-  a.nct = nct;
+  a.ref_order = P.bb_order == MFC_BB_REFERENCE;
+  a.nct = a.ref_order ? 1 : nct;
+  a.ord = P.ord; a.ord_cap = P.ord_cap; a.isn = P.isn[P.map_cur];
   a.nlub = fused ? lub_blocks(c) : 0;
   const int t = P.map_cur ^ 1;
   const int nclr = (fused && P.map_np[t] > 0) ? P.map_np[t] * P.nseg : 0;

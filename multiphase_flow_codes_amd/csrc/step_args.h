@@ -155,6 +155,10 @@ struct BounceArgs {
   int np;
   int split;                    // bounce-back work units per segment (seg_ft has nsegs * split)
   int nct;                      // contact blocks of the pre-pass grid
+  int ref_order;                // contact links in the reference's in-place order (one block)
+  OrderEntry* ord;              // its table (ord_cap entries) and the current owner map
+  int64_t ord_cap;
+  const int32_t* isn;
   int nlub;                     // lubrication blocks of the pre-pass grid (fused step, else 0)
   LubArgs lub;
   int seg_cap;                  // map clearing (clr_* of map buffer map_cur ^ 1)

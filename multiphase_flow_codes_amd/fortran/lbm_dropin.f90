@@ -27,7 +27,10 @@
 ! Precision and storage are read from the environment when the context is
 ! created: MFC_PRECISION = fp64 (default) | fp32 | fp32x (FP32 storage, FP64
 ! collision); MFC_STORAGE = ab (default, two buffers) | aa (one buffer
-! updated in place).
+! updated in place); MFC_BB_ORDER = reference (default: the bounce-back
+! links in this loop's order, each value stored at once) | snapshot (every
+! link against one post-collision field; the two differ only for links whose
+! node is solid for another sphere).
 ! A library error prints 'fault: <routine> (<code>) <message>' as the
 ! reference's own fatal checks do, then ends the run with a nonzero status.
       module mfc_dropin_state
@@ -163,6 +166,15 @@
         error stop 1
       end if
       created = .true.
+      select case (trim(env_choice('MFC_BB_ORDER', 'reference')))
+      case ('reference')
+        call mfc_check(mfc_set_bb_order(ctx, MFC_BB_REFERENCE), 'set_bb_order')
+      case ('snapshot')
+        call mfc_check(mfc_set_bb_order(ctx, MFC_BB_SNAPSHOT), 'set_bb_order')
+      case default
+        write(*,*) 'fault: MFC_BB_ORDER must be reference or snapshot'
+        error stop 1
+      end select
       ! whatever the host holds now is the state (restart: loaded f, ux..uz)
       call mfc_check(mfc_upload_f(ctx, f), 'upload f')
       call mfc_upload_fields(rho, ux, uy, uz, .true., 'upload rho/u')

@@ -11,6 +11,7 @@
       integer(c_int), parameter :: MFC_PREC_FP64 = 0, MFC_PREC_FP32 = 1, &
                                    MFC_PREC_FP32_F64MATH = 2
       integer(c_int), parameter :: MFC_STORE_AB = 0, MFC_STORE_AA = 1
+      integer(c_int), parameter :: MFC_BB_SNAPSHOT = 0, MFC_BB_REFERENCE = 1
       integer(c_int), parameter :: MFC_PF_YPGLB = 0, MFC_PF_YPGLBP = 1, &
         MFC_PF_WP = 2, MFC_PF_WPP = 3, MFC_PF_OMGP = 4, MFC_PF_OMGPP = 5,&
         MFC_PF_FHIP = 6, MFC_PF_FLUBP = 7, MFC_PF_TORQP = 8,            &
@@ -156,6 +157,11 @@
         integer(c_int) function mfc_beads_collision(ctx) bind(C, name="mfc_beads_collision")
           import :: c_int, c_ptr
           type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function mfc_set_bb_order(ctx, order) bind(C, name="mfc_set_bb_order")
+          import :: c_int, c_ptr, c_int32_t
+          type(c_ptr), value :: ctx
+          integer(c_int32_t), value :: order
         end function
         integer(c_int) function mfc_beads_lubforce(ctx) bind(C, name="mfc_beads_lubforce")
           import :: c_int, c_ptr

@@ -29,7 +29,8 @@ PFIELDS = {"ypglb": 0, "ypglbp": 1, "wp": 2, "wpp": 3, "omgp": 4, "omgpp": 5, "f
            "flubp": 7, "torqp": 8, "forcepp": 9, "torqpp": 10, "dwdt": 11, "domgdt": 12,
            "thetap": 13}
 STATUS_NAMES = ["overlap", "overlap_step", "link_overflow", "alpha_fault", "contact_links",
-                "nonfinite", "refill", "nlinks", "nsolid", "box_fault"]
+                "nonfinite", "refill", "nlinks", "nsolid", "box_fault", "cell_overflow", "fast",
+                "order_cap"]
 
 
 class MfcError(RuntimeError):
@@ -92,6 +93,7 @@ SIGNATURES = {
     "mfc_initpartomg": [C.c_void_p],
     "mfc_beads_links": [C.c_void_p],
     "mfc_beads_collision": [C.c_void_p],
+    "mfc_set_bb_order": [C.c_void_p, C.c_int32],
     "mfc_beads_lubforce": [C.c_void_p],
     "mfc_beads_move": [C.c_void_p, C.c_int32],
     "mfc_beads_redistribute": [C.c_void_p, C.c_int32],
@@ -364,6 +366,16 @@ class Lattice:
         ek = np.zeros(int(nek))
         self._chk(lib().mfc_diag_spectrum(self.h, int(nek), _dp(ek)))
         return ek
+
+    def set_bb_order(self, order: str):
+        """Bounce-back link order: "snapshot" (every link against one
+        post-collision field, the default) or "reference" (the reference's
+        one-rank loop order, partlib.f90:1363-1621, which differs only for
+        links whose own node is solid for another sphere)."""
+        codes = {"snapshot": 0, "reference": 1}
+        if order not in codes:
+            raise ValueError(f"bounce-back order must be one of {sorted(codes)}")
+        self._chk(lib().mfc_set_bb_order(self.h, codes[order]))
 
     def set_timing(self, every):
         """Time every `every`-th collide launch with HIP events (True: each

@@ -86,9 +86,10 @@ def read_outputs(workdir, prm):
 
 
 def oracle_run(prm, u, nsteps, nrelax, rhoepsl=1e-30):
-    """BOX/main.f90:73-188 on the oracle, snapshot bounce-back order."""
+    """BOX/main.f90:73-188 on the oracle, in the reference's bounce-back order
+    (the drop-in's default)."""
     from oracle.pyoracle import Oracle
-    orc = Oracle(prm, bb_mode=1)
+    orc = Oracle(prm, bb_mode=0)
     orc.ux[:], orc.uy[:], orc.uz[:] = u
     orc.initpop()
     orc.rhoupdat()

@@ -1,0 +1,234 @@
+"""The reference's bounce-back order on the GPU (mfc_set_bb_order with
+MFC_BB_REFERENCE).  Spheres closer than a node put boundary links on nodes
+solid for the other sphere (contact links); in the reference's one-rank loop
+(BOX/partlib.f90:1363-1621) such a link reads the value an earlier link of the
+(k, j, i, q) loop stored at its node in place, but not one stored later or
+through the halo copy of a periodic y or z face.  Against the FP64 oracle in
+that order (bb_mode 0): whole steps through the single-routine entry points
+and through the fused step, periodic and walled boxes, both storages, a pair
+of spheres whose contact straddles the periodic z face, and 1100 spheres with
+surface gaps from 0.4: populations and particle state within 1e-12 normwise,
+while the snapshot order differs from the oracle's reference order by far
+more than that (the comparison sees the order)."""
+import numpy as np
+import pytest
+
+from helpers import small_params, start_both
+from multiphase_flow_codes_amd.initial import random_spheres
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _pairs(prm):
+    """Two pairs of spheres 0.25 apart: one inside the box, one whose contact
+    straddles the periodic z face."""
+    d = 2 * prm.rad + 0.25
+    e1 = np.array([1.0, 0.4, 0.3]) / np.linalg.norm([1.0, 0.4, 0.3])
+    e2 = np.array([0.2, 0.3, -1.0]) / np.linalg.norm([0.2, 0.3, -1.0])
+    a, c = np.array([6.3, 6.1, 8.2]), np.array([17.2, 15.3, 3.1])
+    pos = np.array([a, a + d * e1, c, c + d * e2])
+    return pos % [prm.nx, prm.ny, prm.nz]
+
+
+def _prm(n=(24, 22, 20), walls=False):
+    kw = dict(forcing=False, strong_forcing=False, body_force=(1e-5, 0.0, 0.0)) if walls else {}
+    return small_params(n=n, npart=4, rad=3.5, rhopart=1.5, walls=walls, **kw)
+
+
+def _start(prm, pos, rng, storage="ab", order="reference"):
+    from multiphase_flow_codes_amd.lbm import Lattice
+    from oracle.pyoracle import Oracle
+    lat = Lattice(prm, "fp64", storage=storage)
+    lat.set_bb_order(order)
+    lat.prt_init(prm.npart)
+    orcs = [Oracle(prm, bb_mode=0), Oracle(prm, bb_mode=1)]
+    state = rng.bit_generator.state
+    for k, orc in enumerate(orcs):
+        rng.bit_generator.state = state
+        if k == 0:
+            start_both(prm, lat, orc, rng, amp=0.02)
+            lat.macrovar()
+        else:
+            from helpers import smooth_velocity
+            orc.ux[:], orc.uy[:], orc.uz[:] = smooth_velocity(prm, rng, 0.02)
+            orc.initpop()
+            orc.rhoupdat()
+        orc.macrovar()
+        orc.ypglb[:] = pos
+        orc.yp[:] = pos
+        orc.S.istep00 = 1
+        orc.call("initpartvel")
+        orc.call("initpartomg")
+        orc.beads_links()
+    lat.prt_set("ypglb", pos)
+    lat.initpartvel()
+    lat.initpartomg()
+    lat.beads_links()
+    return lat, orcs
+
+
+def _check(lat, orcs, it):
+    """The step's bounce-back had contact links (the pairs separate under
+    lubrication within about four steps) and the GPU matches the oracle in
+    the reference order."""
+    ref, snap = orcs
+    assert ref.S.nconflict > 0 and lat.status()["contact_links"] > 0, it
+    f = lat.download_f()
+    assert rel(f, ref.f) < TOL64, it
+    assert rel(snap.f, ref.f) > 1e3 * TOL64, it      # the order matters here
+    for name in ("ypglb", "wp", "omgp", "fHIp", "torqp", "flubp"):
+        assert rel(lat.prt_get(name), getattr(ref, name)) < TOL64, (it, name)
+    ib, isn = lat.get_maps(0)
+    assert np.array_equal(ib, ref.ibnodes) and np.array_equal(isn, ref.isnodes), it
+
+
+@pytest.mark.parametrize("walls", [False, True])
+def test_reference_order_single_routines_match_oracle(oracle_lib, walls):
+    rng = np.random.default_rng(21)
+    prm = _prm(walls=walls)
+    lat, orcs = _start(prm, _pairs(prm), rng)
+    try:
+        for it in (1, 2, 3):
+            for orc in orcs:
+                orc.S.istep = it
+                assert orc.call("timestep") == 0
+            lat.forcing()
+            lat.collision_mrt()
+            lat.beads_collision()
+            lat.streaming()
+            lat.beads_lubforce()
+            lat.beads_move(it)
+            lat.beads_redistribute(it)
+            lat.beads_links()
+            lat.beads_filling()
+            lat.macrovar()
+            lat.sync()
+            _check(lat, orcs, it)
+    finally:
+        lat.close()
+
+
+@pytest.mark.parametrize("walls,storage", [(False, "ab"), (True, "ab"), (False, "aa")])
+def test_reference_order_fused_steps_match_oracle(oracle_lib, walls, storage):
+    rng = np.random.default_rng(22)
+    prm = _prm(walls=walls)
+    lat, orcs = _start(prm, _pairs(prm), rng, storage=storage)
+    try:
+        for it in (1, 2, 3):
+            for orc in orcs:
+                orc.S.istep = it
+                assert orc.call("timestep") == 0
+        lat.step(1, 3)
+        lat.sync()
+        _check(lat, orcs, 3)
+    finally:
+        lat.close()
+
+
+def test_reference_order_many_spheres_match_oracle(oracle_lib):
+    """1100 spheres with surface gaps from 0.4 (more than the 512 above which
+    lubrication uses the cell grid): many contact links, some of whose
+    writers are contact links themselves."""
+    prm = small_params(n=(96, 88, 104), npart=1100, rad=2.5, rhopart=1.5)
+    rng = np.random.default_rng(17)
+    pos = random_spheres(prm.npart, prm.rad, (prm.nz, prm.ny, prm.nx), gap=0.4, seed=17)
+    lat, orcs = _start(prm, pos, rng)
+    try:
+        for it in range(1, 4):
+            for orc in orcs:
+                orc.S.istep = it
+                orc.call("timestep")
+        lat.step(1, 3)
+        lat.sync()
+        _check(lat, orcs, 3)
+    finally:
+        lat.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_reference_order_in_an_ensemble_equals_the_box_alone(oracle_lib, graph):
+    """A box in the reference order with contact links, advanced in an
+    ensemble beside a box in the snapshot order, gives bitwise the results of
+    the same box advanced alone."""
+    from multiphase_flow_codes_amd.lbm import Ensemble
+    prm = _prm(walls=True)
+    boxes = []
+    for _ in range(2):
+        lat, _orcs = _start(prm, _pairs(prm), np.random.default_rng(24))
+        other, _o = _start(_prm(n=(28, 24, 20), walls=True), _pairs(prm),
+                           np.random.default_rng(25), order="snapshot")
+        boxes.append((lat, other))
+    (alone, alone2), (member, member2) = boxes
+    ens = Ensemble()
+    try:
+        ens.set_graph(graph)
+        ens.add(member, 1)
+        ens.add(member2, 1)
+        ens.step(9)
+        ens.sync()
+        alone.step(1, 9)
+        alone2.step(1, 9)
+        alone.sync()
+        alone2.sync()
+        assert member.status()["contact_links"] > 0
+        for a, b in ((alone, member), (alone2, member2)):
+            assert np.array_equal(a.download_f(), b.download_f())
+            for name in ("ypglb", "wp", "omgp", "fHIp", "torqp"):
+                assert np.array_equal(a.prt_get(name), b.prt_get(name)), name
+    finally:
+        ens.close()
+        for pair in boxes:
+            for lat in pair:
+                lat.close()
+
+
+def test_snapshot_and_reference_orders_agree_without_contact(oracle_lib):
+    """With every gap above a node there are no contact links, and the two
+    orders give bitwise the same step."""
+    from multiphase_flow_codes_amd.lbm import Lattice
+    from helpers import place_spheres, smooth_velocity
+    rng = np.random.default_rng(23)
+    prm = small_params(n=(40, 36, 32), npart=6, rad=3.5, rhopart=1.5)
+    pos = place_spheres(prm, rng, prm.npart, gap=2.5)
+    u = smooth_velocity(prm, rng, 0.02)
+    out = []
+    for order in ("snapshot", "reference"):
+        lat = Lattice(prm, "fp32")
+        try:
+            lat.set_bb_order(order)
+            lat.upload_macro(None, *u)
+            lat.initpop()
+            lat.rhoupdat()
+            lat.macrovar()
+            lat.prt_init(prm.npart)
+            lat.prt_set("ypglb", pos)
+            lat.initpartvel()
+            lat.initpartomg()
+            lat.beads_links()
+            lat.step(1, 5)
+            lat.sync()
+            assert lat.status()["contact_links"] == 0
+            out.append((lat.download_f(), lat.prt_get("fHIp"), lat.prt_get("ypglb")))
+        finally:
+            lat.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
+def test_bb_order_argument_checked():
+    from multiphase_flow_codes_amd.lbm import Lattice, MfcError, lib
+    prm = small_params(n=(16, 16, 16), npart=1, rad=3.0)
+    lat = Lattice(prm, "fp64")
+    try:
+        with pytest.raises(ValueError):
+            lat.set_bb_order("in-place")
+        with pytest.raises(MfcError, match="MFC_BB_REFERENCE"):
+            lat._chk(lib().mfc_set_bb_order(lat.h, 7))
+    finally:
+        lat.close()

@@ -571,6 +571,76 @@ def test_bb_reference_order_equals_snapshot_without_contact(oracle_lib):
     assert np.allclose(outs[0][2], outs[1][2], rtol=1e-13, atol=0)
 
 
+def _in_place_readers(o, prm):
+    """Links of the oracle's link map whose own-node reads (f_q at x, and
+    f_qbar at x when alpha > 1/2) see a value another link stored in place
+    earlier in the (k, j, i, q) loop, and what the test sees of the rest:
+    (targets of those links, count of writers that run later, count of
+    earlier writers whose store crosses a periodic y or z face)."""
+    from multiphase_flow_codes_amd.params import IPOPP
+    nx, ny, nz = prm.nx, prm.ny, prm.nz
+    lin = lambda i, j, k: i + nx * (j + ny * k)
+    tainted, later, halo = set(), 0, 0
+    ks, js, is_, qs = np.nonzero(o.ilink > 0)
+    for k, j, i, q0 in zip(ks, js, is_, qs):
+        q = q0 + 1
+        alpha = o.alink[k, j, i, q0]
+        hit = False
+        for sg, qw, used in ((1, IPOPP[q], True), (-1, q, alpha > 0.5)):
+            if not used:
+                continue
+            iw, jw, kw = i + sg * CX[q], j + sg * CY[q], k + sg * CZ[q]
+            direct = 0 <= jw < ny and 0 <= kw < nz
+            iw, jw, kw = iw % nx, jw % ny, kw % nz
+            if o.ilink[kw, jw, iw, qw - 1] <= 0:
+                continue
+            if lin(iw, jw, kw) > lin(i, j, k):
+                later += 1
+            elif not direct:
+                halo += 1
+            else:
+                hit = True
+        if hit:
+            tainted.add(((k + CZ[q]) % nz, (j + CY[q]) % ny, (i + CX[q]) % nx, IPOPP[q]))
+    return tainted, later, halo
+
+
+def test_bb_reference_order_reads_earlier_in_place_stores(oracle_lib):
+    """With spheres 0.25 apart, links sit on nodes solid for the other sphere.
+    The reference order (bb_mode 0) then differs from the snapshot order
+    exactly at the solid slots stored by links that read a value an earlier
+    link of the (k, j, i, q) loop stored in place at their own node; stores
+    that cross the periodic z face go through the halo copy and are not seen,
+    nor are stores of later links.  Spheres without such links keep bitwise
+    the same force and torque."""
+    prm = LBMParams(nx=24, ny=22, nz=20, visc=0.01, npart=4, rad=3.5)
+    d = 2 * prm.rad + 0.25
+    e1 = np.array([1.0, 0.4, 0.3]) / np.linalg.norm([1.0, 0.4, 0.3])
+    e2 = np.array([0.2, 0.3, -1.0]) / np.linalg.norm([0.2, 0.3, -1.0])
+    a, c = np.array([6.3, 6.1, 8.2]), np.array([17.2, 15.3, 3.1])
+    pos = np.array([a, a + d * e1, c, c + d * e2])
+    pos %= [24, 22, 20]
+    outs = []
+    for mode in (0, 1):
+        o = _orc(prm, bb_mode=mode)
+        o.ypglb[:] = pos
+        o.wp[:] = [[0.01, -0.02, 0.005], [0.0, 0.01, 0.0], [-0.01, 0.0, 0.02], [0.0, 0.0, -0.01]]
+        o.omgp[:] = [[0.001, 0.0, -0.002], [0.0, 0.003, 0.0], [0.002, 0.001, 0.0], [0.0, 0.0, 0.004]]
+        o.beads_links()
+        o.f[:] = np.random.default_rng(10).normal(0, 1e-3, o.f.shape)
+        o.beads_collision()
+        assert o.S.nconflict > 20
+        outs.append((o.f.copy(), o.fHIp.copy(), o.torqp.copy(), o))
+    tainted, later, halo = _in_place_readers(outs[0][3], prm)
+    assert len(tainted) > 5 and later > 5 and halo > 0
+    diff = np.argwhere(outs[0][0] != outs[1][0])
+    assert {tuple(int(v) for v in x) for x in diff} == tainted
+    moved = {int(outs[0][3].isnodes[k, j, i]) for (k, j, i, _) in tainted}
+    for p in range(prm.npart):
+        same = np.array_equal(outs[0][1][p], outs[1][1][p]) and np.array_equal(outs[0][2][p], outs[1][2][p])
+        assert same == (p + 1 not in moved), p
+
+
 # -------------------------------------------------------------- physics -----
 def test_shear_wave_decay_rate(oracle_lib):
     """u_x = U sin(2 pi y / L) decays as exp(-nu k^2 t) with
