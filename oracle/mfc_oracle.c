@@ -8,9 +8,10 @@
  * ordering it reproduces.  Where the reference's behaviour depends on its MPI
  * halo copies (beads_collision) the single-rank behaviour is reproduced
  * exactly (bb_mode 0); bb_mode 1 evaluates every bounce-back link against one
- * snapshot of the post-collision field, the order-free form the GPU uses.
- * The two agree whenever no boundary link sits on a node that is solid for
- * another sphere (near contact).
+ * snapshot of the post-collision field (the GPU library's default order; the
+ * GPU's reference order follows bb_mode 0).  The two agree whenever no
+ * boundary link sits on a node that is solid for another sphere (near
+ * contact).
  *
  * What pins each function (tests/test_oracle_ref.py, tests/test_oracle_pins.py,
  * tests/test_oracle_pins_particles.py; their headers give the detail):
@@ -25,7 +26,10 @@
  *   orc_beads_links    brute-force solid and link sets, wall point on the sphere
  *   orc_beads_collision  Lagrange interpolation on quadratic fields (all five
  *                      branches), momentum exchange and torque sums, rotating-
- *                      sphere torque, moving-sphere force closed form
+ *                      sphere torque, moving-sphere force closed form; in
+ *                      bb_mode 0 the slots that differ from bb_mode 1 are
+ *                      exactly those the link map, the loop order and the
+ *                      halo rule predict
  *   orc_beads_lubforce hand-computed magnitudes, symmetry, wall law
  *   orc_beads_move     ramped-force closed form, uniform acceleration
  *   orc_beads_filling  non-equilibrium copy from ipmx on a linear field, tie
