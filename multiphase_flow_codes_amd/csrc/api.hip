@@ -748,7 +748,7 @@ int mfc_prt_init(mfc_ctx* ctx, int32_t npart) {
     if (c.g.nx < need || c.g.nz < need || (!c.g.walls_y && c.g.ny < need))
       return fail(ctx, MFC_ERR_ARG, "box too small for the particle radius (need 2*ceil(rad+2)+6 nodes along x, z and periodic y)");
     if (!(c.p.rad > 0.0)) return fail(ctx, MFC_ERR_ARG, "particle radius must be positive");
-    if ((size_t)segment_capacity(c.p.rad) * sizeof(uint32_t) > 65536)
+    if ((size_t)segment_capacity(c.p.rad, 4) * sizeof(uint32_t) > 65536)
       return fail(ctx, MFC_ERR_ARG, "particle radius too large for the box-scan kernels (rad <= 27)");
   }
   HIPCHK(hipStreamSynchronize(c.stream));
@@ -760,8 +760,9 @@ int mfc_prt_init(mfc_ctx* ctx, int32_t npart) {
       *a = nullptr;
       HIPCHK(hipMalloc(a, sizeof(double) * 3 * npart));
     }
-    P.nseg = segments_per_particle(c.p.rad);
-    P.seg_cap = segment_capacity(c.p.rad);
+    P.slab = segment_planes(c.p.rad, npart);
+    P.nseg = segments_per_particle(c.p.rad, P.slab);
+    P.seg_cap = segment_capacity(c.p.rad, P.slab);
     const size_t nsegs = (size_t)npart * P.nseg;
     const size_t nodes = nsegs * (size_t)P.seg_cap;
     void** segbufs[] = {(void**)&P.lk_cnt, (void**)&P.lk_off, (void**)&P.seg_ft,

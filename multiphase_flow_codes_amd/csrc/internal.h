@@ -119,6 +119,7 @@ struct Particles {
   // the nodes of one segment, so per-segment lists never overflow.
   int nseg = 1;
   int seg_cap = 0;
+  int slab = 4;   // z planes per segment, chosen with the allocation (segment_planes)
   // solid nodes each map build marked, per segment: what the build two steps
   // later clears and what beads_filling scans for uncovered nodes
   uint32_t* sol_list[2] = {nullptr, nullptr};
@@ -231,8 +232,9 @@ hipError_t launch_aos_to_pop(Ctx& c, const double* src, int z0, int nzs, bool pu
 hipError_t launch_force_field(Ctx& c, int slot, double* fx, double* fy, double* fz);
 
 // ---- launchers (particles.hip) ----
-int segments_per_particle(double rad);
-int segment_capacity(double rad);
+int segments_per_particle(double rad, int slab);
+int segment_capacity(double rad, int slab);
+int segment_planes(double rad, int npart);
 // with_find: the first pass of beads_filling (previous map vs the new one)
 // runs in the same grid as the link expansion
 hipError_t launch_links(Ctx& c, int target, int overlap_check, int istep, bool with_find = false);

@@ -23,10 +23,9 @@
 namespace mfc {
 
 constexpr int PB = 256;  // threads per particle workgroup
-// A segment is one z-slab (SLAB planes) of one particle's bounding box: the
-// unit of work of every box-scan kernel, so a sphere spreads over several
-// workgroups.  Segment s of particle p is block p * nseg + s.
-constexpr int SLAB = 4;
+// A segment is one z-slab (G.slab planes, 4 or 8) of one particle's bounding
+// box: the unit of work of every box-scan kernel, so a sphere spreads over
+// several workgroups.  Segment s of particle p is block p * nseg + s.
 
 
 struct AxisRange {
@@ -223,12 +222,12 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* sh, int& total);
 
 // Rows (y, z) of one segment's walk: the x run of each row that can hold
 // nodes within r + 1.5 of the centre (a superset: the node test decides),
-// and the runs' prefix sums.  A segment has at most SLAB * kMaxRowCands rows.
-constexpr int kMaxRowCands = 64;
-static_assert(SLAB * kMaxRowCands <= PB, "one thread per row of a segment");
+// and the runs' prefix sums.  A segment has at most PB rows (one thread per
+// row): slab planes of at most PB / slab candidate rows each, which the
+// choice of slab guarantees (segment_planes).
 struct SegRows {
-  int start[SLAB * kMaxRowCands + 1];   // first flattened index of each row
-  short xa[SLAB * kMaxRowCands];        // first box x index of each row's run
+  int start[PB + 1];   // first flattened index of each row
+  short xa[PB];        // first box x index of each row's run
   int nrows;
 };
 
@@ -240,10 +239,10 @@ struct SegRows {
 template <typename F>
 __device__ __forceinline__ void walk_segment(const BoxInfo& B, const PGeom& G, int s, SegRows& R,
                                              int* sh, F&& fn) {
-  const int z0 = s * SLAB;
-  const int nzs = max(0, min(SLAB, B.az.cnt - z0));
-  const int nrows = nzs * B.ay.cnt;
-  // x run of each row (a row at most PB: SLAB * kMaxRowCands <= PB)
+  const int z0 = s * G.slab;
+  const int nzs = max(0, min(G.slab, B.az.cnt - z0));
+  const int nrows = nzs * B.ay.cnt;   // <= PB
+  // x run of each row (a row at most PB nodes)
   int run = 0, xa = 0;
   const int r = (int)threadIdx.x;
   if (r < nrows) {
@@ -1638,20 +1637,41 @@ static PGeom pgeom(const Ctx& c) {
   G.r2 = (c.p.rad * 1.0) * (c.p.rad * 1.0);
   G.rlink2 = (c.p.rad + 1.5) * (c.p.rad + 1.5);
   G.rsolid2 = (c.p.rad + 0.5) * (c.p.rad + 0.5);
+  G.slab = c.pt.slab;
   return G;
 }
 
-int segments_per_particle(double rad) {
-  // the widest candidate range axis_range() can return, in z-slabs
+// the widest candidate range axis_range() can return, in nodes
+static int box_candidates(double rad) {
   const double radp2 = rad + 2.0;
-  const int maxcnt = (int)ceil(2.0 * radp2 + 1.0) + 4;
-  return (maxcnt + SLAB - 1) / SLAB;
+  return (int)ceil(2.0 * radp2 + 1.0) + 4;
 }
 
-int segment_capacity(double rad) {
-  const double radp2 = rad + 2.0;
-  const int maxcnt = (int)ceil(2.0 * radp2 + 1.0) + 4;
-  return SLAB * maxcnt * maxcnt;
+int segments_per_particle(double rad, int slab) {
+  return (box_candidates(rad) + slab - 1) / slab;
+}
+
+int segment_capacity(double rad, int slab) {
+  const int maxcnt = box_candidates(rad);
+  return slab * maxcnt * maxcnt;
+}
+
+// Planes per segment.  8 when the spheres are many (the segment grids of the
+// link rebuild and the bounce-back still fill the GPU several times over
+// with half as many, twice as large workgroups: at 1024^3 with 20,000
+// spheres of radius 6 the three kernels take 0.5 ms less per step), else 4
+// (300 spheres of radius 8 at 256^3 need the parallelism: 0.06 ms more per
+// step with 8).  8 also needs at most PB / 8 candidate rows per plane and
+// keeps the scan's shell list within 20 KB of LDS, so only for
+// box_candidates <= 25 (rad <= 8).  MFC_SEG_PLANES=4 or 8 forces one (8 only
+// where it fits).
+constexpr int64_t kSlab8MinSegments = 8192;
+int segment_planes(double rad, int npart) {
+  const bool fits8 = box_candidates(rad) <= 25;
+  const char* e = getenv("MFC_SEG_PLANES");
+  if (e && e[0] == '4') return 4;
+  if (e && e[0] == '8') return fits8 ? 8 : 4;
+  return (fits8 && (int64_t)npart * segments_per_particle(rad, 8) >= kSlab8MinSegments) ? 8 : 4;
 }
 
 // Reset map buffer `buf` to empty (no-op if it is empty already).

@@ -34,6 +34,8 @@ struct PGeom {
   double rlink2;
   // (r + 0.5)^2: no solid node lies farther out than this
   double rsolid2;
+  // z planes per segment (4 or 8; Particles::slab)
+  int slab;
 };
 
 struct FillArgs {

@@ -131,10 +131,13 @@ def test_reference_order_fused_steps_match_oracle(oracle_lib, walls, storage):
         lat.close()
 
 
-def test_reference_order_many_spheres_match_oracle(oracle_lib):
+@pytest.mark.parametrize("planes", [None, "8"])
+def test_reference_order_many_spheres_match_oracle(oracle_lib, monkeypatch, planes):
     """1100 spheres with surface gaps from 0.4 (more than the 512 above which
     lubrication uses the cell grid): many contact links, some of whose
-    writers are contact links themselves."""
+    writers are contact links themselves; box segments of 4 and 8 planes."""
+    if planes:
+        monkeypatch.setenv("MFC_SEG_PLANES", planes)
     prm = small_params(n=(96, 88, 104), npart=1100, rad=2.5, rhopart=1.5)
     rng = np.random.default_rng(17)
     pos = random_spheres(prm.npart, prm.rad, (prm.nz, prm.ny, prm.nx), gap=0.4, seed=17)

@@ -2,9 +2,10 @@
 
 Rows longer than 256 nodes are split into several x-chunks, one workgroup
 each (the 1024^3 channel has four per row). The collide's tile order over
-the XCDs (MFC_XCD_REMAP 0, 1 or 2; chosen by shape when unset) and the split
-of a bounce-back segment over several workgroups (MFC_BB_SPLIT) only change
-which workgroup computes which nodes or links. Every setting must match the
+the XCDs (MFC_XCD_REMAP 0, 1 or 2; chosen by shape when unset), the split
+of a bounce-back segment over several workgroups (MFC_BB_SPLIT) and the z
+planes per box segment (MFC_SEG_PLANES 8; 4 unless the spheres are many)
+only change which workgroup computes which nodes or links. Every setting must match the
 oracle to 1e-12 in FP64: the split changes the order of the force sums, so
 1e-12 rather than bitwise. The tile orders must also agree bitwise with each
 other, in FP32 (both arithmetic forms) and in FP64 in-place storage.
@@ -67,7 +68,9 @@ def _oracle_case(monkeypatch, env, nsteps=4):
     {"MFC_XCD_REMAP": "2"},
     {"MFC_BB_SPLIT": "2"},
     {"MFC_BB_SPLIT": "4"},
-], ids=["remap0", "remap1", "remap2", "bbsplit2", "bbsplit4"])
+    {"MFC_SEG_PLANES": "8"},
+    {"MFC_SEG_PLANES": "8", "MFC_BB_SPLIT": "2"},
+], ids=["remap0", "remap1", "remap2", "bbsplit2", "bbsplit4", "seg8", "seg8bbsplit2"])
 def test_wide_rows_match_oracle(oracle_lib, monkeypatch, env):
     lat, orc = _oracle_case(monkeypatch, env)
     assert orc.S.nconflict == 0
