@@ -229,6 +229,15 @@ def run_serving_bench(ncases, cap, rate, window=500, tol=1e-4):
         r = runs[prec]
         log(f"serving {prec}: p50 {r['p50_s']:.2f} s, p99 {r['p99_s']:.2f} s, {r['cases_per_hour']:.0f} cases/h, "
             f"peak footprint {r['peak_footprint_bytes'] / 1e9:.1f} GB, waits {r['admission_waits']}")
+    # the same stream under a budget of a quarter of the FP32 engine's peak
+    # footprint (at least the largest case): admission waits, scheduling
+    # changes, results do not
+    tight_budget = max(0.25 * runs["fp32"]["peak_footprint_bytes"],
+                       max(c.footprint_bytes("fp32") for c in cases))
+    tight = serving.Server("fp32", budget_bytes=tight_budget, window=window, tol=tol, cap=cap).run(cases)
+    log(f"serving fp32 under {tight_budget / 1e9:.2f} GB: p50 {tight['p50_s']:.2f} s, p99 {tight['p99_s']:.2f} s, "
+        f"{tight['cases_per_hour']:.0f} cases/h, peak footprint {tight['peak_footprint_bytes'] / 1e9:.2f} GB, "
+        f"waits {tight['admission_waits']}")
     one = serving.run_one_at_a_time(cases, "fp64", window=window, tol=tol, cap=cap)
     log(f"one-at-a-time fp64: p50 {one['p50_s']:.2f} s, p99 {one['p99_s']:.2f} s, "
         f"{one['cases_per_hour']:.0f} cases/h")
@@ -258,6 +267,11 @@ def run_serving_bench(ncases, cap, rate, window=500, tol=1e-4):
                         f"run until mean drag changes < {tol} over {window} steps, cap {cap} steps, "
                         f"240 GB admission budget",
             "engine_fp32": fmt(runs["fp32"]), "engine_fp64": fmt(runs["fp64"]),
+            "engine_fp32_tight_budget": fmt(tight) | {
+                "budget_bytes": tight_budget,
+                "peak_within_budget": tight["peak_footprint_bytes"] <= tight_budget,
+                "drag_bitwise_equal_to_unbounded_engine": all(
+                    tight["drag_history"][c.seed] == runs["fp32"]["drag_history"][c.seed] for c in cases)},
             "one_at_a_time_fp64": fmt(one),
             "cases_per_hour_ratio_fp32_engine": round(runs["fp32"]["cases_per_hour"] / one["cases_per_hour"], 3),
             "cases_per_hour_ratio_fp64_engine": round(runs["fp64"]["cases_per_hour"] / one["cases_per_hour"], 3),
