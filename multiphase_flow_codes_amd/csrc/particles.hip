@@ -1343,9 +1343,13 @@ __device__ __forceinline__ void fill_apply_body(int b_, T* __restrict__ A, Grid 
               v9 = v9 + (double)cy(ip) * f9[ip];
               w9 = w9 + (double)cz(ip) * f9[ip];
             }
-            double feq[NPOP];
-            feq_point(u9, v9, w9, rho9, fa.ww0, fa.ww1, fa.ww2, feq);
-            for (int ip = 0; ip < NPOP; ++ip) fneq[w][ip] = f9[ip] - feq[ip];
+            // feq_point's expressions one direction at a time (no array of 19)
+            const double usqr = 1.5 * (u9 * u9 + v9 * v9 + w9 * w9);
+            fneq[w][0] = f9[0] - fa.ww0 * (rho9 - usqr);
+            for (int ip = 1; ip < NPOP; ++ip) {
+              const double G = (double)cx(ip) * u9 + (double)cy(ip) * v9 + (double)cz(ip) * w9;
+              fneq[w][ip] = f9[ip] - (ip <= 6 ? fa.ww1 : fa.ww2) * (rho9 + 3.0 * G + 4.5 * G * G - usqr);
+            }
           }
         }
         srho[w][lane] = s;
@@ -1555,8 +1559,11 @@ __global__ void __launch_bounds__(256)
   move_body((int)blockIdx.x - off[c], a.m, a.np, a.first, a.wrap, a.G, a.status, a.seg_ft, a.nseg);
 }
 
+// 4 waves per SIMD (at most 128 VGPRs): with the 1024 blocks of the grid
+// every wavefront is resident at once
 template <typename T, bool WALLS>
-__global__ void __launch_bounds__(256) k_fill_apply(FillApplyArgs a, TableJob tj) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+    k_fill_apply(FillApplyArgs a, TableJob tj) {
   if ((int)blockIdx.x >= a.nblk) {   // blocks past the refill: the next step's forcing table
     force_table_block(tj.M, tj.phy, tj.phz, tj.ny, tj.nz, tj.G, blockIdx.x - (uint32_t)a.nblk);
     return;
@@ -1565,7 +1572,7 @@ __global__ void __launch_bounds__(256) k_fill_apply(FillApplyArgs a, TableJob tj
 }
 static_assert(sizeof(FillApplyArgs) + sizeof(TableJob) <= 4096, "refill kernel arguments over 4 KB");
 template <typename T, bool WALLS>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
     kb_fill_apply(const FillApplyArgs* __restrict__ t, const int* __restrict__ off, int nc) {
   const int c = case_of(off, nc, (int)blockIdx.x);
   const FillApplyArgs a = t[c];
