@@ -154,6 +154,29 @@ def test_reference_order_many_spheres_match_oracle(oracle_lib, monkeypatch, plan
         lat.close()
 
 
+@pytest.mark.parametrize("mode", ["1", "2"])
+def test_reference_order_overlapped_step_is_bitwise_the_plain_step(oracle_lib, monkeypatch, mode):
+    """The overlapped step (particle phase on a second stream beside the far
+    tiles' collide) with contact links in the reference order gives bitwise
+    the plain step's populations and particle state."""
+    prm = _prm()
+    plain, _o1 = _start(prm, _pairs(prm), np.random.default_rng(26))
+    monkeypatch.setenv("MFC_OVERLAP", mode)
+    over, _o2 = _start(prm, _pairs(prm), np.random.default_rng(26))
+    monkeypatch.delenv("MFC_OVERLAP")
+    try:
+        for lat in (plain, over):
+            lat.step(1, 3)
+            lat.sync()
+        assert over.status()["contact_links"] > 0
+        assert np.array_equal(plain.download_f(), over.download_f())
+        for name in ("ypglb", "wp", "omgp", "fHIp", "torqp", "flubp"):
+            assert np.array_equal(plain.prt_get(name), over.prt_get(name)), name
+    finally:
+        plain.close()
+        over.close()
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_reference_order_in_an_ensemble_equals_the_box_alone(oracle_lib, graph):
     """A box in the reference order with contact links, advanced in an
