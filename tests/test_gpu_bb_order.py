@@ -41,10 +41,10 @@ def _prm(n=(24, 22, 20), walls=False):
     return small_params(n=n, npart=4, rad=3.5, rhopart=1.5, walls=walls, **kw)
 
 
-def _start(prm, pos, rng, storage="ab", order="reference"):
+def _start(prm, pos, rng, storage="ab", order="reference", precision="fp64"):
     from multiphase_flow_codes_amd.lbm import Lattice
     from oracle.pyoracle import Oracle
-    lat = Lattice(prm, "fp64", storage=storage)
+    lat = Lattice(prm, precision, storage=storage)
     lat.set_bb_order(order)
     lat.prt_init(prm.npart)
     orcs = [Oracle(prm, bb_mode=0), Oracle(prm, bb_mode=1)]
@@ -150,6 +150,29 @@ def test_reference_order_many_spheres_match_oracle(oracle_lib, monkeypatch, plan
         lat.step(1, 3)
         lat.sync()
         _check(lat, orcs, 3)
+    finally:
+        lat.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp32x"])
+def test_reference_order_in_fp32_storage_follows_the_fp64_oracle(oracle_lib, precision):
+    """FP32 storage in the reference order: within FP32 round-off of the FP64
+    oracle's reference order, and a hundred times closer to it than the
+    snapshot order is."""
+    prm = _prm()
+    lat, (ref, snap) = _start(prm, _pairs(prm), np.random.default_rng(27), precision=precision)
+    try:
+        for it in (1, 2, 3):
+            for orc in (ref, snap):
+                orc.S.istep = it
+                assert orc.call("timestep") == 0
+        lat.step(1, 3)
+        lat.sync()
+        assert lat.status()["contact_links"] > 0
+        err = rel(lat.download_f(), ref.f)
+        assert err < 1e-5
+        assert err < 1e-2 * rel(snap.f, ref.f)
+        assert rel(lat.prt_get("fHIp"), ref.fHIp) < 1e-4
     finally:
         lat.close()
 
