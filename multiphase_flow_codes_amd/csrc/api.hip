@@ -370,15 +370,29 @@ int mfc_create(const mfc_params* p, mfc_ctx** out) {
     c.overlap = (e && (e[0] == '1' || e[0] == '2')) ? e[0] - '0' : 0;
   }
   {
-    // Population planes 512 B further apart than the node count: with a
-    // power-of-two plane size every one of the 38 streams of the collide
-    // would start at the same offset of each memory page.  512 B measured
-    // 1.5-2 % faster than no padding on the 256^3 FP32/FP64 and 1024^3 FP32
-    // collides (neutral for in-place FP64, 4 KB slower for FP64).
-    // MFC_PLANE_PAD overrides it in 256-B units (0: no padding).
-    const char* e = getenv("MFC_PLANE_PAD");
-    const long pad = e ? atol(e) : 2;
-    if (pad >= 0 && pad < (1L << 20)) c.g.pitch = ncell + (uint64_t)pad * (256 / c.tsize);
+    // Population layout.  By default the 19 planes' rows of one (y, z) lie
+    // side by side (pitch = nx, rs = 19 nx): a collide workgroup's 38
+    // streams then fall in a few neighbouring pages instead of 38 planes
+    // 4 GB apart.  Measured against the plane-major layout, alternating in
+    // one box: FP32 collide 2.6 % faster at 1024^3 and 8 % at 256^3 (step
+    // 2.5 % and 5 %), FP64 in place 2.3 % faster at 1024^3, FP64 and FP32
+    // storage with FP64 collision unchanged at 256^3; padding the plane
+    // stride or the row group was slower in a probe of the access pattern.
+    // MFC_POP_LAYOUT=planes selects plane-major planes instead, 512 B
+    // further apart than the node count (a power-of-two plane size would
+    // start every stream at the same offset of each page; 512 B measured
+    // 1.5-2 % faster than no padding there); MFC_PLANE_PAD overrides that
+    // pad in 256-B units.
+    const char* e = getenv("MFC_POP_LAYOUT");
+    if (e && e[0] == 'p') {
+      const char* d = getenv("MFC_PLANE_PAD");
+      const long pad = d ? atol(d) : 2;
+      if (pad >= 0 && pad < (1L << 20)) c.g.pitch = ncell + (uint64_t)pad * (256 / c.tsize);
+      c.g.rs = (uint64_t)c.g.nx;
+    } else {
+      c.g.pitch = (uint64_t)c.g.nx;
+      c.g.rs = (uint64_t)NPOP * c.g.nx;
+    }
   }
   c.iseedf = p->iseedf;
   int rc = MFC_OK;
@@ -393,7 +407,9 @@ int mfc_create(const mfc_params* p, mfc_ctx** out) {
   } while (0)
   CK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   c.own_stream = true;
-  const size_t pb = (size_t)19 * c.g.pitch * c.tsize;
+  // one past the last element: plane 18, the last row, x = nx - 1
+  const size_t pelems = (size_t)(NPOP - 1) * c.g.pitch + (size_t)(ncell / c.g.nx - 1) * c.g.rs + c.g.nx;
+  const size_t pb = std::max(pelems, (size_t)NPOP * (c.g.rs == (uint64_t)c.g.nx ? c.g.pitch : 0)) * c.tsize;
   CK(hipMalloc(&c.pop[0], pb));
   CK(hipMemsetAsync(c.pop[0], 0, pb, c.stream));
   if (c.g.lay == LAY_AB) {

@@ -1,7 +1,9 @@
 // fluid.hip -- lattice kernels of libmfc_lbm for gfx950.
 //
-// Storage: populations are SoA planes pop[i * pitch + n], n = x + nx*(y + ny*z)
-// (pitch = the node count plus 512 B), in FP64 or FP32 (deviations
+// Storage: populations are SoA, direction i of node (x, y, z) at
+// pop[i * pitch + x + rs * (y + ny*z)]: by default the 19 directions' rows of
+// one (y, z) side by side (pitch = nx, rs = 19 nx), else whole planes
+// (rs = nx, pitch = the node count plus 512 B); in FP64 or FP32 (deviations
 // f_i - w_i*rho0).  The resident buffer holds the
 // post-collision (pre-streaming) field; streaming is never a separate pass:
 // every consumer pulls f_i(x) = buf[i][x - c_i] (half-way bounce-back at the
@@ -99,20 +101,21 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
   const uint32_t zz[3] = {zp, (uint32_t)z, zm};   // cz = -1, 0, +1
   const uint32_t xm = x == 0 ? nx - 1 : x - 1, xpp = x == g.nx - 1 ? 0 : x + 1;
   const uint32_t xs[3] = {xpp, (uint32_t)x, xm};   // cx = -1, 0, +1
-  const uint32_t n = (uint32_t)x + nx * row;
+  const uint32_t n = (uint32_t)x + nx * row;            // node index (mask, moment arrays)
+  const uint64_t own = (uint64_t)x + g.rs * row;        // in-plane offset of the node
   const bool wall_lo = WALLS && y == 0, wall_hi = WALLS && y == g.ny - 1;
   // location of the streamed value of direction i: plane offset (pl * pitch,
   // a select between two compile-time planes on the wave-uniform wall test,
-  // so it stays a scalar) and slot
-  auto src_of = [&](int i, uint64_t& pbase, uint32_t& src) {
+  // so it stays a scalar) and in-plane offset
+  auto src_of = [&](int i, uint64_t& pbase, uint64_t& src) {
     if (LAY == LAY_AA_ODD) {
       pbase = (uint64_t)i * g.pitch;
-      src = n;
+      src = own;
       return;
     }
     const int ccx = cx(i), ccy = cy(i), ccz = cz(i);
     const bool wall = (ccy > 0 && wall_lo) || (ccy < 0 && wall_hi);
-    src = wall ? n : xs[ccx + 1] + nx * (yy[ccy + 1] + ny * zz[ccz + 1]);
+    src = wall ? own : xs[ccx + 1] + g.rs * (yy[ccy + 1] + ny * zz[ccz + 1]);
     const int p_in = LAY == LAY_AB ? i : opp(i);       // plane of an interior pull
     const int p_wall = LAY == LAY_AB ? opp(i) : i;     // plane beyond a wall
     const uint64_t b_in = (uint64_t)p_in * g.pitch, b_wall = (uint64_t)p_wall * g.pitch;
@@ -122,7 +125,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
 #pragma unroll
   for (int i = 0; i < NPOP; ++i) {
     uint64_t pb;
-    uint32_t src;
+    uint64_t src;
     src_of(i, pb, src);
     // two buffers: the field read is not read again this step (non-temporal)
     f[i] = LAY == LAY_AB ? (R)__builtin_nontemporal_load(&A[pb + src]) : (R)F[pb + src];
@@ -170,7 +173,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
   }
   if (LAY == LAY_AB) {
     // one running pointer over the planes: no per-plane 64-bit base to keep
-    T* q = B + n;
+    T* q = B + own;
 #pragma unroll
     for (int i = 0; i < NPOP; ++i) {
       __builtin_nontemporal_store((T)f[i], q);
@@ -180,7 +183,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
 #pragma unroll
     for (int i = 0; i < NPOP; ++i) {
       uint64_t pb;
-      uint32_t dst;
+      uint64_t dst;
       src_of(opp(i), pb, dst);
       __builtin_nontemporal_store((T)f[i], &F[pb + dst]);
     }

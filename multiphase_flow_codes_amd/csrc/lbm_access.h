@@ -20,63 +20,69 @@ __device__ __forceinline__ void node_coords(uint32_t n, const Grid& g, int& x, i
   z = (int)zz;
 }
 
-__device__ __forceinline__ uint32_t node_wrapped(const Grid& g, int x, int y, int z) {
+// Offset of node (x, y, z) within a plane (the plane's own element is
+// plane * pitch + this).
+__device__ __forceinline__ uint64_t node_off(const Grid& g, int x, int y, int z) {
+  return (uint64_t)x + g.rs * ((uint64_t)y + (uint64_t)g.ny * (uint64_t)z);
+}
+
+__device__ __forceinline__ uint64_t node_off_wrapped(const Grid& g, int x, int y, int z) {
   const int wx = x < 0 ? x + g.nx : (x >= g.nx ? x - g.nx : x);
   const int wy = y < 0 ? y + g.ny : (y >= g.ny ? y - g.ny : y);
   const int wz = z < 0 ? z + g.nz : (z >= g.nz ? z - g.nz : z);
-  return (uint32_t)wx + (uint32_t)g.nx * ((uint32_t)wy + (uint32_t)g.ny * (uint32_t)wz);
+  return node_off(g, wx, wy, wz);
 }
 
-// Location (plane, slot) of the post-collision value of direction i at node
-// n = (x,y,z) in layout `lay`.
+// Location (plane, in-plane offset) of the post-collision value of direction
+// i at node (x,y,z) in layout `lay`.
 template <bool WALLS>
-__device__ __forceinline__ void post_src(int lay, int i, int x, int y, int z, uint32_t n,
-                                         const Grid& g, int& plane, uint32_t& slot) {
+__device__ __forceinline__ void post_src(int lay, int i, int x, int y, int z, const Grid& g,
+                                         int& plane, uint64_t& slot) {
   if (lay == LAY_AB) {
     plane = i;
-    slot = n;
+    slot = node_off(g, x, y, z);
     return;
   }
   if (lay == LAY_AA_EVEN) {
     plane = opp(i);
-    slot = n;
+    slot = node_off(g, x, y, z);
     return;
   }
   const int ty = y + cy(i);
   if (WALLS && (ty < 0 || ty >= g.ny)) {
     plane = opp(i);
-    slot = n;
+    slot = node_off(g, x, y, z);
     return;
   }
   plane = i;
-  slot = node_wrapped(g, x + cx(i), ty, z + cz(i));
+  slot = node_off_wrapped(g, x + cx(i), ty, z + cz(i));
 }
 
 // Location of the streamed value f_i at node (x,y,z): the post-collision value
 // of i at x - c_i, or beyond a wall that of opp(i) at the node itself.
 template <bool WALLS>
-__device__ __forceinline__ void pull_src_lay(int lay, int i, int x, int y, int z, uint32_t n,
-                                             const Grid& g, int& plane, uint32_t& slot) {
+__device__ __forceinline__ void pull_src_lay(int lay, int i, int x, int y, int z, const Grid& g,
+                                             int& plane, uint64_t& slot) {
   if (lay == LAY_AA_ODD) {   // x - c_i + c_i = x, and the wall image lands there too
     plane = i;
-    slot = n;
+    slot = node_off(g, x, y, z);
     return;
   }
   const int sy = y - cy(i);
   const bool wall = WALLS && (sy < 0 || sy >= g.ny);
   if (wall) {
     plane = lay == LAY_AB ? opp(i) : i;
-    slot = n;
+    slot = node_off(g, x, y, z);
     return;
   }
   plane = lay == LAY_AB ? i : opp(i);
-  slot = node_wrapped(g, x - cx(i), sy, z - cz(i));
+  slot = node_off_wrapped(g, x - cx(i), sy, z - cz(i));
 }
 
 template <bool WALLS>
-__device__ __forceinline__ void pull_src(int i, int x, int y, int z, uint32_t n, const Grid& g,
-                                         int& plane, uint32_t& src) {
-  pull_src_lay<WALLS>(g.lay, i, x, y, z, n, g, plane, src);
+__device__ __forceinline__ void pull_src(int i, int x, int y, int z, const Grid& g, int& plane,
+                                         uint64_t& src) {
+  pull_src_lay<WALLS>(g.lay, i, x, y, z, g, plane, src);
 }
 
 template <typename T, typename R, bool WALLS>
@@ -85,8 +91,8 @@ __device__ __forceinline__ void load_pulled(const T* A, const Grid& g, int x, in
 #pragma unroll
   for (int i = 0; i < NPOP; ++i) {
     int pl;
-    uint32_t s;
-    pull_src<WALLS>(i, x, y, z, n, g, pl, s);
+    uint64_t s;
+    pull_src<WALLS>(i, x, y, z, g, pl, s);
     f[i] = (R)A[(uint64_t)pl * g.pitch + s];
   }
 }
@@ -99,8 +105,8 @@ __device__ __forceinline__ void load_direct(const T* A, const Grid& g, int x, in
 #pragma unroll
   for (int i = 0; i < NPOP; ++i) {
     int pl;
-    uint32_t s;
-    post_src<WALLS>(g.lay, i, x, y, z, n, g, pl, s);
+    uint64_t s;
+    post_src<WALLS>(g.lay, i, x, y, z, g, pl, s);
     f[i] = (R)A[(uint64_t)pl * g.pitch + s];
   }
 }
@@ -112,27 +118,28 @@ template <typename T, bool WALLS>
 __device__ __forceinline__ void store_phase(T* A, const Grid& g, int i, int x, int y, int z,
                                             uint32_t n, bool pulled, double v) {
   int pl;
-  uint32_t s;
+  uint64_t s;
   if (pulled)
-    pull_src<WALLS>(i, x, y, z, n, g, pl, s);
+    pull_src<WALLS>(i, x, y, z, g, pl, s);
   else
-    post_src<WALLS>(g.lay, i, x, y, z, n, g, pl, s);
+    post_src<WALLS>(g.lay, i, x, y, z, g, pl, s);
   A[(uint64_t)pl * g.pitch + s] = (T)v;
 }
 
 // Value of direction i at node n after collision, read by the bounce-back.
 template <typename T>
 __device__ __forceinline__ double post_value(const T* A, const Grid& g, int i, uint32_t n) {
-  if (g.lay == LAY_AB) return (double)A[(uint64_t)i * g.pitch + n];
+  // plane-major two buffers: the node index is the in-plane offset
+  if (g.lay == LAY_AB && g.rs == (uint64_t)g.nx) return (double)A[(uint64_t)i * g.pitch + n];
   int x, y, z;
   uint32_t row;
   node_coords(n, g, x, y, z, row);
   int pl;
-  uint32_t s;
+  uint64_t s;
   if (g.walls_y)
-    post_src<true>(g.lay, i, x, y, z, n, g, pl, s);
+    post_src<true>(g.lay, i, x, y, z, g, pl, s);
   else
-    post_src<false>(g.lay, i, x, y, z, n, g, pl, s);
+    post_src<false>(g.lay, i, x, y, z, g, pl, s);
   return (double)A[(uint64_t)pl * g.pitch + s];
 }
 

@@ -70,7 +70,13 @@ struct Grid {
   int walls_y;        // 1: half-way bounce-back walls on the y faces
   int lay;            // PopLayout of the resident buffer
   uint64_t ncell;     // nx*ny*nz
-  uint64_t pitch;     // elements from one population plane to the next (>= ncell)
+  // Population element of (plane q, node x + nx*(y + ny*z)):
+  //   q * pitch + x + rs * (y + ny*z).
+  // Plane-major layout: pitch >= ncell, rs = nx (each plane a contiguous
+  // field).  Row-interleaved layout: pitch = nx, rs = 19*nx (the 19 planes'
+  // rows of one (y, z) side by side).
+  uint64_t pitch;     // elements from one population plane to the next
+  uint64_t rs;        // elements from one row of a plane to the next row
 };
 
 MFC_HD int wrapi(int a, int n) {

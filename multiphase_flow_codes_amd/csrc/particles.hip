@@ -1022,13 +1022,12 @@ __device__ __forceinline__ void bounce_back_body(int b_, const BounceArgs& a) {
     const int yt = y + cy(q);
     if (!(g.walls_y && (yt < 0 || yt >= g.ny))) {
       const int xt = wrap1(x + cx(q), g.nx), zt = wrap1(z + cz(q), g.nz), ytw = wrap1(yt, g.ny);
-      const uint32_t t = (uint32_t)xt + (uint32_t)g.nx * ((uint32_t)ytw + (uint32_t)g.ny * (uint32_t)zt);
       int pl;
-      uint32_t s;
+      uint64_t s;
       if (g.walls_y)
-        post_src<true>(g.lay, opp(q), xt, ytw, zt, t, g, pl, s);
+        post_src<true>(g.lay, opp(q), xt, ytw, zt, g, pl, s);
       else
-        post_src<false>(g.lay, opp(q), xt, ytw, zt, t, g, pl, s);
+        post_src<false>(g.lay, opp(q), xt, ytw, zt, g, pl, s);
       A[(uint64_t)pl * g.pitch + s] = (T)f9;
     }
     const double dff = ff1 + f9;

@@ -84,8 +84,8 @@ class BoxCase:
 
     def footprint_bytes(self, precision: str) -> int:
         """Device memory make_lattice allocates for the case: the sizes
-        mfc_create and mfc_prt_init request (populations A/B with the plane
-        pad, forcing tables, moments, particle arrays, segment lists, link
+        mfc_create and mfc_prt_init request (populations A/B, 19 values per
+        node each, forcing tables, moments, particle arrays, segment lists, link
         pool, refill list, node maps), each rounded up to the allocator's
         granularity (4 KiB below 1 MiB, 2 MiB above), plus 16 MiB for the
         runtime's per-context state.  An upper bound of the hipMemGetInfo
@@ -93,8 +93,7 @@ class BoxCase:
         t = 8 if precision == "fp64" else 4
         n, np_, r = self.n, self.npart, self.rad
         ncell = n ** 3
-        pitch = ncell + 512 // t
-        allocs = [19 * pitch * t] * 2 + [8 * 18 * n * n] * 4 + [8 * 6 * n, 64, 64] + [8 * ncell] * 4
+        allocs = [19 * ncell * t] * 2 + [8 * 18 * n * n] * 4 + [8 * 6 * n, 64, 64] + [8 * ncell] * 4
         if np_ > 0:
             allocs += [8 * 3 * np_] * 16
             maxcnt = math.ceil(2.0 * (r + 2.0) + 1.0) + 4

@@ -157,21 +157,28 @@ def test_aa_refused_by_ensemble():
         lat.close()
 
 
-@pytest.mark.parametrize("storage", ["ab", "aa"])
-def test_padded_plane_pitch_is_bitwise_neutral(storage, monkeypatch):
-    """Population planes spaced wider than the node count (MFC_PLANE_PAD,
-    in 256-B units) change addresses only: fused steps with spheres and
-    forcing give bitwise the same field and sphere state."""
+@pytest.mark.parametrize("storage,precision", [("ab", "fp64"), ("aa", "fp64"), ("ab", "fp32")])
+def test_population_layouts_are_bitwise_neutral(storage, precision, monkeypatch):
+    """The row-interleaved population layout (the default), plane-major
+    planes (MFC_POP_LAYOUT=planes) and plane-major planes spaced wider than
+    the node count (MFC_PLANE_PAD, in 256-B units) change addresses only:
+    fused steps with spheres and forcing give bitwise the same field and
+    sphere state."""
     name = "periodic"
     prm = small_params(**CASES[name])
     pos = _positions(name, prm, np.random.default_rng(3))
     out = []
-    for pad in ("0", "17"):
-        monkeypatch.setenv("MFC_PLANE_PAD", pad)
-        lat = _released(prm, storage, "fp64", pos, 7)
+    for layout, pad in ((None, None), ("planes", "0"), ("planes", "17")):
+        for var, val in (("MFC_POP_LAYOUT", layout), ("MFC_PLANE_PAD", pad)):
+            if val is None:
+                monkeypatch.delenv(var, raising=False)
+            else:
+                monkeypatch.setenv(var, val)
+        lat = _released(prm, storage, precision, pos, 7)
         lat.step(1, 5)
         lat.sync()
         out.append((lat.download_f(), lat.prt_get("ypglb"), lat.prt_get("fHIp")))
         lat.close()
-    for a, b in zip(*out):
-        assert np.array_equal(a, b)
+    for other in out[1:]:
+        for a, b in zip(out[0], other):
+            assert np.array_equal(a, b)
