@@ -79,7 +79,11 @@ __device__ __forceinline__ void force_row(const double* __restrict__ t, R c1, R 
 // node touches exactly the locations it read and nodes never race.  From
 // LAY_AA_ODD that is all in the node's own slots; from LAY_AA_EVEN it is the
 // neighbours' (the result is in LAY_AA_ODD, and the other way round).
-template <typename T, typename R, bool ARR, bool WALLS, int LAY>
+// NTL: the two-buffer field is read with non-temporal loads (it is not read
+// again this step); with rows split over several x-chunks plain loads let the
+// chunks share the cache lines at their edges (1024^3: 1.2 % faster collide;
+// whole rows at 256^3: 3-8 % slower with plain loads).
+template <typename T, typename R, bool ARR, bool WALLS, int LAY, bool NTL = true>
 __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y, int z) {
   const T* __restrict__ A = (const T*)a.A;
   T* __restrict__ B = (T*)a.B;
@@ -127,8 +131,8 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
     uint64_t pb;
     uint64_t src;
     src_of(i, pb, src);
-    // two buffers: the field read is not read again this step (non-temporal)
-    f[i] = LAY == LAY_AB ? (R)__builtin_nontemporal_load(&A[pb + src]) : (R)F[pb + src];
+    f[i] = LAY == LAY_AB ? (NTL ? (R)__builtin_nontemporal_load(&A[pb + src]) : (R)A[pb + src])
+                         : (R)F[pb + src];
   }
   const bool solid = mask != nullptr && ((mask[n >> 5] >> (n & 31u)) & 1u);
   if (!solid) {
@@ -225,7 +229,7 @@ __device__ __forceinline__ void xcd_tile(uint32_t L, uint32_t total, uint32_t nb
 // FP32 storage with FP64 arithmetic, two buffers: 4 waves per SIMD (128
 // VGPRs, 20-28 B/lane of spills) instead of 3 (132 VGPRs) measured 6 % faster
 // at 256^3; the same bound on the FP64 kernel made it 4 % slower.
-template <typename T, typename R, bool ARR, bool WALLS, int LAY>
+template <typename T, typename R, bool ARR, bool WALLS, int LAY, bool NTL>
 __global__ void __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && sizeof(R) == 8 && LAY == LAY_AB ? 4 : 1)))
     k_collide(CollideArgs a, int nbx, int remap) {
@@ -234,7 +238,7 @@ __global__ void __launch_bounds__(256)
   const int x = (int)(bx * blockDim.x + threadIdx.x);
   if (x >= a.g.nx) return;
   const uint32_t y = row % (uint32_t)a.g.ny, z = row / (uint32_t)a.g.ny;
-  collide_node<T, R, ARR, WALLS, LAY>(a, x, (int)y, (int)z);
+  collide_node<T, R, ARR, WALLS, LAY, NTL>(a, x, (int)y, (int)z);
 }
 
 // The collide of the tiles of a list (overlapped step): one tile (kTileX
@@ -523,6 +527,22 @@ CollideArgs collide_args(Ctx& c) {
   return a;
 }
 
+template <typename T, typename R, int LAY, bool NTL>
+static void collide_launch(Ctx& c, bool arrays, const CollideArgs& a, dim3 grid, dim3 blk, int nbx,
+                           int rm) {
+  if (arrays) {
+    if (c.g.walls_y)
+      hipLaunchKernelGGL((k_collide<T, R, true, true, LAY, NTL>), grid, blk, 0, c.stream, a, nbx, rm);
+    else
+      hipLaunchKernelGGL((k_collide<T, R, true, false, LAY, NTL>), grid, blk, 0, c.stream, a, nbx, rm);
+  } else {
+    if (c.g.walls_y)
+      hipLaunchKernelGGL((k_collide<T, R, false, true, LAY, NTL>), grid, blk, 0, c.stream, a, nbx, rm);
+    else
+      hipLaunchKernelGGL((k_collide<T, R, false, false, LAY, NTL>), grid, blk, 0, c.stream, a, nbx, rm);
+  }
+}
+
 template <typename T, typename R, int LAY>
 static void collide_dispatch_lay(Ctx& c, bool arrays, const CollideArgs& a) {
   dim3 grid, blk;
@@ -533,17 +553,13 @@ static void collide_dispatch_lay(Ctx& c, bool arrays, const CollideArgs& a) {
   // (measured best for each: at 256^3 FP64 2.3 % faster step with ranges,
   // FP32 2.7 % slower)
   const int rm = c.xcd_remap >= 0 ? c.xcd_remap : ((nbx > 1 || sizeof(T) == 8) ? 1 : 0);
-  if (arrays) {
-    if (c.g.walls_y)
-      hipLaunchKernelGGL((k_collide<T, R, true, true, LAY>), grid, blk, 0, c.stream, a, nbx, rm);
-    else
-      hipLaunchKernelGGL((k_collide<T, R, true, false, LAY>), grid, blk, 0, c.stream, a, nbx, rm);
-  } else {
-    if (c.g.walls_y)
-      hipLaunchKernelGGL((k_collide<T, R, false, true, LAY>), grid, blk, 0, c.stream, a, nbx, rm);
-    else
-      hipLaunchKernelGGL((k_collide<T, R, false, false, LAY>), grid, blk, 0, c.stream, a, nbx, rm);
+  if constexpr (LAY == LAY_AB) {
+    if (nbx > 1) {   // rows in several x-chunks: plain loads
+      collide_launch<T, R, LAY, false>(c, arrays, a, grid, blk, nbx, rm);
+      return;
+    }
   }
+  collide_launch<T, R, LAY, true>(c, arrays, a, grid, blk, nbx, rm);
 }
 
 template <typename T, typename R>
