@@ -35,9 +35,9 @@ BYTES_PER_CELL = {"fp32": 152, "fp32x": 152, "fp64": 304}   # 19 populations rea
 # factors a plain 1 GiB device copy gives); in-place storage: the mean of its two
 # alternating layouts
 TRAFFIC_SOURCE = "profiles/hbm_traffic_box256_channel1024.md"
-MEASURED_TRAFFIC = {("box256", "fp32", "ab"): 2.64635e9, ("box256", "fp64", "ab"): 5.18949e9,
-                    ("box256", "fp32x", "ab"): 2.63511e9,
-                    ("channel1024", "fp32", "ab"): 1.68750e11, ("channel1024", "fp64", "aa"): 3.29678e11}
+MEASURED_TRAFFIC = {("box256", "fp32", "ab"): 2.64561e9, ("box256", "fp64", "ab"): 5.17563e9,
+                    ("box256", "fp32x", "ab"): 2.63475e9,
+                    ("channel1024", "fp32", "ab"): 1.63341e11, ("channel1024", "fp64", "aa"): 3.29672e11}
 KERNEL = {"fp32": "k_collide<float,float>", "fp32x": "k_collide<float,double>",
           "fp64": "k_collide<double,double>"}
 
