@@ -128,3 +128,95 @@ def sphere_counts(pos, r, n):
             inside = np.roll(solid, (-CIZ[q], -CIY[q], -CIX[q]), axis=(1, 2, 3))   # solid(x + c)
             nlinks += int(np.count_nonzero(fluid & inside))
     return nsolid, nlinks, np.concatenate(solid_idx)
+
+
+# --- the compiled reference slab driver (oracle/_ref/ref_driver) ---------------
+
+REF_NX = 128        # lx of the reference's var_inc.f90:14, a compile-time constant
+
+
+def hash_field(shape, seed, scale):
+    """Values in [-scale/2, scale/2) from a splitmix64 hash of the flat index
+    and the seed: exact in float64 and independent of numpy's generators, so
+    a fixture stores only outputs and its inputs are rebuilt bit for bit."""
+    n = int(np.prod(shape))
+    g = np.uint64(0x9E3779B97F4A7C15)
+    with np.errstate(over="ignore"):
+        z = np.arange(1, n + 1, dtype=np.uint64) * g + np.uint64(seed) * np.uint64(0xD1B54A32D192ED03)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 2.0 ** 53)
+    return ((u - 0.5) * scale).reshape(shape)
+
+
+def slab_params(ly, lz, npart=0):
+    """Parameters of the reference's own scaling for an lx = 128 slab of
+    ly x lz rows (the driver takes ly and lz at run time)."""
+    prm = LBMParams.from_reference_scaling(REF_NX, npart=npart)
+    prm.ny, prm.nz = ly, lz
+    return prm
+
+
+def hashed_state(o, seed, solid_frac=0.1):
+    """Fluid state for the slab tests: populations, density and velocity
+    deviations, forcing and a random solid-node flag, all from hash_field."""
+    shp = o.rho.shape
+    o.f[:] = hash_field(o.f.shape, seed, 7e-3)
+    o.rho[:] = hash_field(shp, seed + 1000, 3.5e-3)
+    for k, n in enumerate(("ux", "uy", "uz")):
+        getattr(o, n)[:] = hash_field(shp, seed + 2000 + k, 0.1)
+    for k, n in enumerate(("fx", "fy", "fz")):
+        getattr(o, n)[:] = hash_field(shp, seed + 3000 + k, 3.5e-4)
+    u = hash_field(shp, seed + 4000, 1.0) + 0.5
+    o.ibnodes[:] = np.where(u < solid_frac, 1, -1)
+
+
+def run_ref_driver(driver, tmp_dir, mode, ly, lz, f, rho, ux, uy, uz, fx, fy, fz, ibn,
+                   isn=None, ypglb=None, wp=None, omgp=None):
+    """Runs the compiled reference on one slab state.  mode 1: collision_MRT,
+    2: macrovar, 3: rhoupdat, 4: the para constants.  Returns f, rho, ux, uy,
+    uz after the call and the scalar block (para constants)."""
+    import os
+    import subprocess
+    fin, fout = os.path.join(str(tmp_dir), "in.bin"), os.path.join(str(tmp_dir), "out.bin")
+    np_ = 0 if ypglb is None else len(ypglb)
+    with open(fin, "wb") as fh:
+        np.array([mode, ly, lz, np_], dtype=np.int32).tofile(fh)
+        for a in (f, rho, ux, uy, uz, fx, fy, fz):
+            np.ascontiguousarray(a, dtype=np.float64).tofile(fh)
+        np.ascontiguousarray(ibn, dtype=np.int32).tofile(fh)
+        if mode == 2:
+            np.ascontiguousarray(isn, dtype=np.int32).tofile(fh)
+            if np_:
+                for a in (ypglb, wp, omgp):
+                    np.ascontiguousarray(a, dtype=np.float64).tofile(fh)
+    subprocess.check_call([driver, fin, fout])
+    out = np.fromfile(fout, dtype=np.float64)
+    nc = REF_NX * ly * lz
+    f_o = out[:19 * nc].reshape(lz, ly, REF_NX, 19)
+    rest = out[19 * nc:]
+    rho_o, ux_o, uy_o, uz_o = (rest[k * nc:(k + 1) * nc].reshape(lz, ly, REF_NX) for k in range(4))
+    return f_o, rho_o, ux_o, uy_o, uz_o, rest[4 * nc:]
+
+
+PARA_NAMES = ["tau", "s1", "s2", "s4", "s9", "s10", "s13", "s16", "omegepsl", "omegepslj", "omegxx",
+              "ww0", "ww1", "ww2", "visc", "vscale", "tscale", "pi2"]
+
+
+def sphere_slab(o, rng_seed, cen, rad=1.5):
+    """Solid nodes of spheres (centres cen, radius rad) in a slab state, with
+    hashed particle velocities; the centres keep the spheres clear of the
+    slab faces (no periodic images)."""
+    npart = len(cen)
+    o.ypglb[:] = cen
+    o.wp[:] = hash_field((npart, 3), rng_seed, 2e-2)
+    o.omgp[:] = hash_field((npart, 3), rng_seed + 1, 2e-3)
+    lz, ly, lx = o.rho.shape
+    zz, yy, xx = np.meshgrid(np.arange(lz) + 0.5, np.arange(ly) + 0.5, np.arange(lx) + 0.5, indexing="ij")
+    o.ibnodes[:] = -1
+    o.isnodes[:] = -1
+    for p, c in enumerate(cen):
+        m = (xx - c[0]) ** 2 + (yy - c[1]) ** 2 + (zz - c[2]) ** 2 <= rad ** 2
+        o.ibnodes[m] = 1
+        o.isnodes[m] = p + 1

@@ -9,47 +9,26 @@ pin the oracle's collision_MRT, macrovar, rhoupdat and the constants of para
 Skipped where the reference build is absent.
 """
 import os
-import subprocess
 
 import numpy as np
 import pytest
 
-from multiphase_flow_codes_amd.params import LBMParams
+from helpers import PARA_NAMES, REF_NX as NX, run_ref_driver
+from helpers import slab_params as _slab_params
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
-NX, LY, LZ = 128, 8, 8
+LY, LZ = 8, 8
 
 pytestmark = pytest.mark.skipif(not os.path.exists(REF), reason="reference build (oracle/_ref) absent")
 
 
-def run_ref(tmp_path, mode, f, rho, ux, uy, uz, fx, fy, fz, ibn, isn=None, ypglb=None, wp=None, omgp=None):
-    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
-    np_ = 0 if ypglb is None else len(ypglb)
-    with open(fin, "wb") as fh:
-        np.array([mode, LY, LZ, np_], dtype=np.int32).tofile(fh)
-        for a in (f, rho, ux, uy, uz, fx, fy, fz):
-            np.ascontiguousarray(a, dtype=np.float64).tofile(fh)
-        np.ascontiguousarray(ibn, dtype=np.int32).tofile(fh)
-        if mode == 2:
-            np.ascontiguousarray(isn, dtype=np.int32).tofile(fh)
-            if np_:
-                for a in (ypglb, wp, omgp):
-                    np.ascontiguousarray(a, dtype=np.float64).tofile(fh)
-    subprocess.check_call([REF, fin, fout])
-    out = np.fromfile(fout, dtype=np.float64)
-    nc = NX * LY * LZ
-    f_o = out[:19 * nc].reshape(LZ, LY, NX, 19)
-    rest = out[19 * nc:]
-    rho_o, ux_o, uy_o, uz_o = (rest[k * nc:(k + 1) * nc].reshape(LZ, LY, NX) for k in range(4))
-    scal = rest[4 * nc:]
-    return f_o, rho_o, ux_o, uy_o, uz_o, scal
+def run_ref(tmp_path, mode, *arrays, **kw):
+    return run_ref_driver(REF, tmp_path, mode, LY, LZ, *arrays, **kw)
 
 
 def slab_params(npart=0):
-    prm = LBMParams.from_reference_scaling(128, npart=npart)
-    prm.ny, prm.nz = LY, LZ
-    return prm
+    return _slab_params(LY, LZ, npart=npart)
 
 
 def random_state(o, rng, solid_frac=0.1):
@@ -67,9 +46,7 @@ def test_para_constants_match_reference(tmp_path, oracle_lib):
     prm = slab_params()
     o = Oracle(prm)
     *_, scal = run_ref(tmp_path, 4, o.f, o.rho, o.ux, o.uy, o.uz, o.fx, o.fy, o.fz, o.ibnodes)
-    names = ["tau", "s1", "s2", "s4", "s9", "s10", "s13", "s16", "omegepsl", "omegepslj", "omegxx",
-             "ww0", "ww1", "ww2", "visc", "vscale", "tscale", "pi2"]
-    for name, v in zip(names, scal):
+    for name, v in zip(PARA_NAMES, scal):
         ours = getattr(prm, name)
         assert ours == v, f"{name}: host mirror {ours!r} != reference {v!r}"
 
@@ -118,7 +95,8 @@ def test_macrovar_with_solid_nodes_matches_reference(tmp_path, oracle_lib):
         o.isnodes[m] = p + 1
     assert (o.ibnodes > 0).sum() > 20
     _, rho_r, ux_r, uy_r, uz_r, _ = run_ref(tmp_path, 2, o.f, o.rho, o.ux, o.uy, o.uz, o.fx, o.fy,
-                                            o.fz, o.ibnodes, o.isnodes, o.ypglb, o.wp, o.omgp)
+                                            o.fz, o.ibnodes, isn=o.isnodes, ypglb=o.ypglb, wp=o.wp,
+                                            omgp=o.omgp)
     o.macrovar()
     for a, b in ((o.rho, rho_r), (o.ux, ux_r), (o.uy, uy_r), (o.uz, uz_r)):
         assert np.max(np.abs(a - b)) <= 1e-18 + 4e-16 * np.max(np.abs(b))
