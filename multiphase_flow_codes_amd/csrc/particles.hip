@@ -305,16 +305,38 @@ __device__ __forceinline__ bool link_hit(const LinkGeom& L, int iw, int jw, int 
   return (rr1 - r2) <= 0.0;
 }
 
-// Exclusive prefix sum over the PB threads of a block: a shuffle scan inside
+// Wavefront-wide inclusive scans through DPP lane moves (no LDS round trip):
+// within each row of 16 lanes by shifts of 1, 2, 4 and 8 (lanes shifted in
+// from outside the row contribute 0), then lane 15 of each row into the next
+// row and lane 31 into the upper half.  Every lane of the wavefront must be
+// active.  DPP controls: row_shr:n = 0x110 + n, row_bcast:15 = 0x142,
+// row_bcast:31 = 0x143; rows left out by the row mask keep the old value 0.
+__device__ __forceinline__ int wave_inclusive_add(int x) {
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);
+  return x;
+}
+
+// Bitwise OR of x over the wavefront (every lane active), in every lane.
+__device__ __forceinline__ uint32_t wave_or(uint32_t x) {
+  x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);
+  x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);
+  x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);
+  x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);
+  x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);
+  x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+
+// Exclusive prefix sum over the PB threads of a block: a DPP scan inside
 // each 64-lane wavefront, then the wavefront totals through LDS.
 __device__ __forceinline__ int block_exclusive_scan(int v, int* sh, int& total) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int x = v;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int y = __shfl_up(x, off, 64);
-    if (lane >= off) x += y;
-  }
+  const int x = wave_inclusive_add(v);
   if (lane == 63) sh[w] = x;
   __syncthreads();
   int before = 0, tot = 0;
@@ -341,11 +363,9 @@ __device__ __forceinline__ void mask_update(uint32_t* __restrict__ mask, bool on
   const int lane = threadIdx.x & 63;
   while (pending) {
     const int leader = __ffsll((unsigned long long)pending) - 1;
-    const uint32_t wl = __shfl(wd, leader, 64);
+    const uint32_t wl = (uint32_t)__builtin_amdgcn_readlane((int)wd, leader);
     const bool mem = on && wd == wl;
-    uint32_t v = mem ? bit : 0u;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+    const uint32_t v = wave_or(mem ? bit : 0u);
     if (lane == leader) {
       if (set)
         atomicOr(&mask[wl], v);
@@ -544,11 +564,11 @@ __device__ __forceinline__ void scan_counts_body(int b_, const int32_t* __restri
 __device__ void fill_find_segment(int b, const FillArgs& fa, int nseg, int32_t* __restrict__ status);
 
 // Second half of beads_links: expand each segment's (node, direction bits)
-// into links at the segment's offset, in (node, direction) order, then the
-// wall fraction of every link of the
-// segment (partlib.f90:1013-1024), one link per lane, with the node geometry
-// rebuilt exactly as the scan saw it.  Blocks past the link segments run the
-// first pass of beads_filling (the new map is complete at this point).
+// into links at the segment's offset, in (node, direction) order, with the
+// wall fraction of every link (partlib.f90:1013-1024), one link per lane and
+// the node geometry rebuilt exactly as the scan saw it.  Blocks past the link
+// segments run the first pass of beads_filling (the new map is complete at
+// this point).
 __device__ __forceinline__ void links_expand_body(int b_, const double* __restrict__ ypglb, int np, int nseg, int seg_cap, PGeom G,
                    const uint32_t* __restrict__ sh_node, const uint32_t* __restrict__ sh_hits,
                    const int32_t* __restrict__ sh_cnt, const int64_t* __restrict__ off, int64_t cap,
@@ -1267,7 +1287,7 @@ __device__ void fill_find_segment(int b, const FillArgs& fa, int nseg, int32_t* 
     const int leader = __ffsll((unsigned long long)bal) - 1;
     int slot0 = 0;
     if (lane == leader) slot0 = atomicAdd(fa.count, __popcll(bal));
-    slot0 = __shfl(slot0, leader, 64);
+    slot0 = __builtin_amdgcn_readlane(slot0, leader);
     if (fill) {
       const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
       const int slot = slot0 + __popcll(bal & below);
