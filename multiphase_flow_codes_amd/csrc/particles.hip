@@ -846,6 +846,7 @@ __device__ __forceinline__ void link_bounce(const T* A, const Grid& g, const PGe
 // so the dependencies are acyclic and pass m settles chains of length m).
 // The writer of f_q(x) is the link of x's owner at x + c_q along -c_q, the
 // writer of f_qbar(x) its link at x - c_q along c_q.
+constexpr int kOrderBatch = 16;
 template <typename T>
 __device__ void bb_contact_ordered(const BounceArgs& a) {
   __shared__ int changed;
@@ -886,12 +887,21 @@ __device__ void bb_contact_ordered(const BounceArgs& a) {
         const int qw = s == 0 ? opp(q) : q;
         const int64_t lo = a.off[(int64_t)own * a.nseg];
         const int64_t hi = min(a.off[(int64_t)(own + 1) * a.nseg], a.cap);
+        // the first match, kOrderBatch independent loads per round trip
         int64_t lw = -1;
-        for (int64_t k = lo; k < hi; ++k)
-          if (a.lk_node[k] == nw && a.lk_dir[k] == qw) {
-            lw = k;
-            break;
+        for (int64_t k0 = lo; k0 < hi && lw < 0; k0 += kOrderBatch) {
+          uint32_t nn[kOrderBatch];
+          int dd[kOrderBatch];
+#pragma unroll
+          for (int j = 0; j < kOrderBatch; ++j) {
+            const bool in = k0 + j < hi;
+            nn[j] = in ? a.lk_node[k0 + j] : ~0u;
+            dd[j] = in ? (int)a.lk_dir[k0 + j] : -1;
           }
+#pragma unroll
+          for (int j = 0; j < kOrderBatch; ++j)
+            if (lw < 0 && nn[j] == nw && dd[j] == qw) lw = k0 + j;
+        }
         if (lw < 0) continue;
         if (bit_at(a.mask, nw)) {
           o.dep[s] = lw | kOrderContact;
