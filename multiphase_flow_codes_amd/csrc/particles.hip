@@ -722,7 +722,12 @@ __device__ __forceinline__ bool lub_pair(const double* __restrict__ ypglb, int a
 }
 
 // beads_lubforce of particle a from the cells: partners in the 27 cells
-// around it, summed in increasing id (a fixed order); then the walls.
+// around it, summed in increasing id (a fixed order); then the walls.  One
+// pass over the cells lists the partners in range, sorted and each once (a
+// cell grid of fewer than 3 cells on an axis meets a cell twice); only a
+// sphere with more than kLubList partners falls back to finding each next
+// id by a further pass.
+constexpr int kLubList = 32;
 __device__ void lub_cells(int a, const double* __restrict__ ypglb, int np, const PGeom& G,
                           const LubArgs& L, int32_t* __restrict__ status) {
   if (a >= np) return;
@@ -732,6 +737,44 @@ __device__ void lub_cells(int a, const double* __restrict__ ypglb, int np, const
   int ca, cb, cc;
   cell_coords(ypglb, a, G, C, ca, cb, cc);
   double sx = 0.0, sy = 0.0, sz = 0.0;
+  int ids[kLubList];
+  int m = 0;
+  bool full = false;
+  for (int k = 0; k < 27; ++k) {
+    const int cell = cell_neighbour(C, G, ca, cb, cc, k);
+    if (cell < 0) continue;
+    const int n = min(C.cnt[cell], kCellCap);
+    for (int e = 0; e < n; ++e) {
+      const int b = C.list[cell * kCellCap + e];
+      if (b == a) continue;
+      double fx, fy, fz;
+      if (!lub_pair(ypglb, a, b, G, L, pre, fx, fy, fz)) continue;
+      int pos = m;
+      while (pos > 0 && ids[pos - 1] > b) --pos;
+      if (pos > 0 && ids[pos - 1] == b) continue;   // met through a repeated cell
+      if (m == kLubList) {
+        full = true;
+        continue;
+      }
+      for (int i = m; i > pos; --i) ids[i] = ids[i - 1];
+      ids[pos] = b;
+      ++m;
+    }
+  }
+  if (!full) {
+    for (int i = 0; i < m; ++i) {
+      double fx, fy, fz;
+      lub_pair(ypglb, a, ids[i], G, L, pre, fx, fy, fz);
+      sx += fx;
+      sy += fy;
+      sz += fz;
+    }
+    if (G.walls_y) sy += wall_lub(ypglb[3 * a + 1], G, L);
+    L.flubp[3 * a] = sx;
+    L.flubp[3 * a + 1] = sy;
+    L.flubp[3 * a + 2] = sz;
+    return;
+  }
   int last = -1;
   for (int guard = 0; guard < 27 * kCellCap; ++guard) {
     int next = 0x7fffffff;
